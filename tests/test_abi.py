@@ -1,0 +1,98 @@
+"""C-ABI library checks that need no GPU: it loads, exports every symbol the
+headers declare, keeps the reference's struct/enum layout, and rejects bad
+arguments before touching a device."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from tridsolver_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INCLUDE = os.path.join(ROOT, "include")
+
+
+def _declared_symbols():
+    names = set()
+    for fn in os.listdir(INCLUDE):
+        if not fn.endswith(".h"):
+            continue
+        text = open(os.path.join(INCLUDE, fn)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"EXTERN_C\s+[\w\s\*]+?\b(\w+)\s*\(", text):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_built():
+    assert os.path.exists(_lib.LIBPATH), "run python -m tridsolver_amd.build"
+
+
+def test_every_declared_symbol_is_exported_and_bound():
+    declared = _declared_symbols()
+    assert {"tridDmtsvStridedBatch", "tridSmtsvStridedBatch", "tridDmtsvStridedBatchInc",
+            "tridSmtsvStridedBatchInc"} <= declared
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIBPATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
+    missing = declared - exported
+    assert not missing, missing
+    assert declared == set(_lib.SIGNATURES), declared ^ set(_lib.SIGNATURES)
+    lib = _lib.lib()
+    for name in declared:
+        assert getattr(lib, name) is not None
+
+
+def test_struct_and_enum_layout():
+    # struct TridParams {void*; int opts[3]; int sync;} is 24 bytes on LP64
+    # (reference scalar/include/trid_params.h:31-35).
+    assert C.sizeof(_lib.TridParams) == 24
+    assert _lib.TridParams.opts.offset == 8 and _lib.TridParams.sync.offset == 20
+    hdr = open(os.path.join(INCLUDE, "trid_common.h")).read()
+    vals = dict((m.group(1), int(m.group(2)))
+                for m in re.finditer(r"(TRID_STATUS_\w+)\s*=\s*(\d+)", hdr))
+    assert vals == {v: k for k, v in _lib.STATUS.items()}
+
+
+def _call_invalid(**kw):
+    lib = _lib.lib()
+    args = dict(ndim=3, solvedim=0, dims=[4, 4, 4], pads=[4, 4, 4], ptr=C.c_void_p(16))
+    args.update(kw)
+    p = args["ptr"]
+    dims = None if args["dims"] is None else _lib.int3(args["dims"])
+    pads = None if args["pads"] is None else _lib.int3(args["pads"])
+    return lib.tridDmtsvStridedBatch(None, p, p, p, p, args["ndim"], args["solvedim"], dims,
+                                     pads)
+
+
+@pytest.mark.parametrize("kw", [
+    dict(ndim=0), dict(ndim=4), dict(solvedim=3), dict(solvedim=-1),
+    dict(dims=[4, -1, 4]), dict(pads=[3, 4, 4]), dict(dims=None), dict(pads=None),
+    dict(ptr=None),
+])
+def test_invalid_arguments_rejected_without_device(kw):
+    assert _call_invalid(**kw) == 3  # TRID_STATUS_INVALID_VALUE
+
+
+def test_inc_requires_u():
+    lib = _lib.lib()
+    p = C.c_void_p(16)
+    st = lib.tridSmtsvStridedBatchInc(None, p, p, p, p, None, 2, 1, _lib.int3([4, 4]),
+                                      _lib.int3([4, 4]))
+    assert st == 3
+
+
+def test_workspace_bytes():
+    lib = _lib.lib()
+    assert lib.tridHipWorkspaceBytes(1, 0, 3, 1, _lib.int3([8, 4, 2])) == 8 * 8 * 4 * 2
+    assert lib.tridHipWorkspaceBytes(0, 1, 3, 1, _lib.int3([8, 4, 2])) == 2 * 4 * 8 * 4 * 2
+    assert lib.tridHipWorkspaceBytes(1, 0, 0, 0, _lib.int3([8])) == 0
+
+
+def test_force_hybrid_config_validation():
+    lib = _lib.lib()
+    assert lib.tridHipForceHybridConfig(-1, 1, 1) == 3
+    assert lib.tridHipForceHybridConfig(8, 0, 1) == 3
+    assert lib.tridHipForceHybridConfig(0, 0, 0) == 0

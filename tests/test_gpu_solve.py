@@ -1,0 +1,210 @@
+"""GPU parity of the batched solve through the C ABI (libtridhip.so) against
+the oracle on identical seeded inputs, and against the reference-format
+fixtures.
+
+Tolerances (written here, used everywhere below):
+  normwise  max|x - x_oracle| / max|x_oracle| <= 1e-12 (FP64), 1e-5 (FP32);
+  elementwise, the reference tests' formula |e-a| <= atol + rtol*min(|e|,|a|)
+  with atol = rtol = 1e-11 (FP64), 1e-6 / 1e-5 (FP32)
+  (scalar/test/utils.hpp:14-17).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from tests import helpers as H
+from tests.golden import gen_fixtures as G
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+from tridsolver_amd import api  # noqa: E402
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    torch.cuda.set_device(0)
+    yield
+    torch.cuda.synchronize()
+
+
+def dev(x):
+    return torch.from_numpy(np.ascontiguousarray(x)).cuda()
+
+
+def gpu_solve(a, b, c, d, u=None, *, solvedim, dims, pads=None, offset=0,
+              opts=(0, 0, 0), sync=True):
+    pads = dims if pads is None else pads
+    ta, tb, tc, td = (dev(t) for t in (a, b, c, d))
+    tu = dev(u) if u is not None else None
+    api.trid_strided_batch(ta, tb, tc, td, solvedim=solvedim, dims=dims, pads=pads, u=tu,
+                           offset=offset, opts=opts, sync=sync)
+    if not sync:
+        torch.cuda.synchronize()
+    return td.cpu().numpy(), (tu.cpu().numpy() if tu is not None else None)
+
+
+ALGOS = {"auto": (0, 0, 0), "lane": (1, 1, 1), "hybrid": (3, 3, 3)}
+
+
+def run_case(dims, solvedim, dtype, inc, opts, seed, garbage_ends=True):
+    rng = np.random.default_rng(seed)
+    n = int(np.prod(dims))
+    a, b, c, d = H.diag_dominant(rng, n, dtype)
+    if garbage_ends:
+        # a[0] and c[N-1] of every system must be ignored; give them values that
+        # would visibly corrupt any result that used them.
+        for idx in H.system_slices(len(dims), solvedim, dims):
+            a[idx[0]] = 1e3
+            c[idx[-1]] = -1e3
+    u = rng.uniform(-1, 1, n).astype(dtype) if inc else None
+    gd, gu = gpu_solve(a, b, c, d, u, solvedim=solvedim, dims=dims, opts=opts)
+    od, ou = d.copy(), (u.copy() if inc else None)
+    oracle.batch(a, b, c, od, ou, ndim=len(dims), solvedim=solvedim, dims=dims, inc=inc)
+    if inc:
+        assert np.array_equal(gd, d), "Inc must not write d"
+        H.assert_norm_close(gu - u, ou - u, dtype)
+        H.ref_allclose(ou, gu, dtype)
+    else:
+        H.assert_norm_close(gd, od, dtype)
+        H.ref_allclose(od, gd, dtype)
+    return api.last_config()
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("name", sorted(G.COMMITTED))
+def test_reference_fixtures(name, dtype):
+    m = G.load_case("tests/golden/files/" + name, dtype)
+    for opts in ALGOS.values():
+        gd, _ = gpu_solve(m["a"], m["b"], m["c"], m["d"], solvedim=m["solvedim"],
+                          dims=m["dims"], opts=opts)
+        H.ref_allclose(m["u"], gd, dtype)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_reference_fixtures_3d(tmp_path, dtype):
+    cases = G.reference_cases(str(tmp_path), G.ON_DEMAND)
+    for name, path in sorted(cases.items()):
+        m = G.load_case(path, dtype)
+        for opts in ALGOS.values():
+            gd, _ = gpu_solve(m["a"], m["b"], m["c"], m["d"], solvedim=m["solvedim"],
+                              dims=m["dims"], opts=opts)
+            H.ref_allclose(m["u"], gd, dtype)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("name", sorted(G.COMMITTED))
+def test_reference_fixtures_padded(name, dtype):
+    m = G.load_case("tests/golden/files/" + name, dtype)
+    dims = m["dims"]
+    arrs = [H.pad_halo(m[k], dims) for k in "abcd"]
+    off, pads = H.halo_offset(dims), [t + 2 for t in dims]
+    for opts in ALGOS.values():
+        gd, _ = gpu_solve(*arrs, solvedim=m["solvedim"], dims=dims, pads=pads, offset=off,
+                          opts=opts)
+        want = H.pad_halo(m["u"], dims)
+        H.ref_allclose(want, gd, dtype)
+
+
+@pytest.mark.parametrize("algo", sorted(ALGOS))
+@pytest.mark.parametrize("inc", [False, True])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("dims", [[37, 21, 19], [64, 64, 64], [128, 24, 40], [33],
+                                  [700], [40, 17], [5, 300]])
+def test_random_vs_oracle(dims, dtype, inc, algo):
+    for sd in range(len(dims)):
+        run_case(dims, sd, dtype, inc, ALGOS[algo], seed=len(dims) * 100 + sd)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("M", [1, 2, 4, 8, 16])
+@pytest.mark.parametrize("W", [1, 3, 16])
+def test_forced_hybrid_shapes(M, W, dtype):
+    """Every compiled M, several W, lengths that are not multiples of M."""
+    for dims, sd in (([61, 5, 7], 0), ([19, 45, 3], 1), ([17, 4, 51], 2)):
+        N = dims[sd]
+        P = -(-N // M)
+        if W * P > (512 if np.dtype(dtype).itemsize * M > 64 else 1024):
+            continue
+        with api.forced_hybrid(M, P, W):
+            cfg = run_case(dims, sd, dtype, False, (0, 0, 0), seed=M * 7 + W)
+        assert cfg == (3, M, P, W)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_edge_lengths(dtype):
+    for dims in ([1], [2], [1, 9], [9, 1], [2, 3, 1], [3000], [2500, 2]):
+        for sd in range(len(dims)):
+            for algo in ("lane", "hybrid", "auto"):
+                run_case(dims, sd, dtype, False, ALGOS[algo], seed=11)
+                run_case(dims, sd, dtype, True, ALGOS[algo], seed=12)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_padded_random(dtype):
+    """Interior of a halo-padded allocation; the halo must stay untouched."""
+    rng = np.random.default_rng(21)
+    dims = [30, 22, 14]
+    n = int(np.prod(dims))
+    a, b, c, d = H.diag_dominant(rng, n, dtype)
+    arrs = [H.pad_halo(t, dims) for t in (a, b, c, d)]
+    off, pads = H.halo_offset(dims), [t + 2 for t in dims]
+    for sd in range(3):
+        for algo in ALGOS.values():
+            gd, _ = gpu_solve(*arrs, solvedim=sd, dims=dims, pads=pads, offset=off,
+                              opts=algo)
+            od = arrs[3].copy()
+            oracle.batch(arrs[0], arrs[1], arrs[2], od, ndim=3, solvedim=sd, dims=dims,
+                         pads=pads, offset=off)
+            H.assert_norm_close(gd, od, dtype)
+            mask = H.pad_halo(np.ones(n, dtype=bool), dims)
+            assert np.array_equal(gd[~mask], arrs[3][~mask])
+
+
+def test_async_on_torch_stream():
+    rng = np.random.default_rng(5)
+    dims = [64, 32, 16]
+    n = int(np.prod(dims))
+    a, b, c, d = H.diag_dominant(rng, n)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        ta, tb, tc, td = (dev(t) for t in (a, b, c, d))
+        api.trid_strided_batch(ta, tb, tc, td, solvedim=1, dims=dims, sync=False)
+        api.trid_strided_batch(ta, tb, tc, td, solvedim=2, dims=dims, sync=False)
+    s.synchronize()
+    od = d.copy()
+    oracle.batch(a, b, c, od, ndim=3, solvedim=1, dims=dims)
+    oracle.batch(a, b, c, od, ndim=3, solvedim=2, dims=dims)
+    H.assert_norm_close(td.cpu().numpy(), od)
+    api.use_torch_stream(torch.cuda.default_stream())
+
+
+@pytest.mark.parametrize("solvedim", [0, 1, 2])
+def test_config_64cube_fp64(solvedim):
+    """64^3 FP64 along x, y, z: the GPU matches the reference CPU library
+    (or the oracle where it is not built) to 1e-12; errors vs the
+    long-double solve are reported."""
+    rng = np.random.default_rng(64)
+    dims = [64, 64, 64]
+    n = 64 ** 3
+    a, b, c, d = H.diag_dominant(rng, n)
+    gd, _ = gpu_solve(a, b, c, d, solvedim=solvedim, dims=dims)
+    ref = d.copy()
+    if oracle.ref_available():
+        oracle.ref_batch(a, b, c, ref, ndim=3, solvedim=solvedim, dims=dims)
+    else:
+        oracle.batch(a, b, c, ref, ndim=3, solvedim=solvedim, dims=dims)
+    err = H.assert_norm_close(gd, ref, np.float64)
+    xl = oracle.batch_ld(a, b, c, d, ndim=3, solvedim=solvedim, dims=dims)
+    print("64^3 sd=%d: vs CPU %.2e, GPU vs long double %.2e, CPU vs long double %.2e"
+          % (solvedim, err, H.norm_rel_err(gd, xl), H.norm_rel_err(ref, xl)))
+
+
+def test_invalid_arguments_on_device():
+    t = torch.zeros(8, dtype=torch.float64, device="cuda")
+    with pytest.raises(Exception):
+        api.trid_strided_batch(t, t, t, t, solvedim=1, dims=[8], pads=[8])
+    with pytest.raises(Exception):
+        api.trid_strided_batch(t, t, t, t, solvedim=0, dims=[4, 3], pads=[4, 3])

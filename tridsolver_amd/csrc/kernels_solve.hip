@@ -1,0 +1,456 @@
+// kernels_solve.hip - gfx950 kernels of the batched scalar tridiagonal solve.
+//
+// Two algorithms:
+//
+// 1. lane_kernel: one Thomas recurrence per lane (wave64 lanes = 64 systems).
+//    For strided systems (solvedim 1/2) neighbouring lanes own x-neighbouring
+//    systems, so every row load of a wave is one contiguous 64*sizeof(REAL)
+//    segment. The forward coefficients c', d' go to a workspace laid out
+//    [row][system] (coalesced), d' in place in d for the plain solve. Any
+//    length. Arithmetic order of the reference CPU Thomas
+//    (scalar/src/cpu/trid_cpu.cpp:312-353): bb = 1/b0, cc = bb*c0, dd = bb*d0;
+//    bb = 1/(b - a*cc), cc = bb*c, dd = bb*(d - a*dd); x = d' - c'*x.
+//    This is the reference's trid_strided_multidim / trid_linear role
+//    (scalar/src/cuda/trid_strided_multidim.hpp:43-189, trid_linear.hpp:40-109).
+//
+// 2. hybrid_kernel: Thomas-PCR hybrid with the whole system on chip, one HBM
+//    read of a,b,c,d and one write of the result. The reference's
+//    batchedTrid_{contig,strided}_ker (scalar/src/cuda/trid_thomaspcr.hpp:121-235,
+//    308-504) has the same three phases; the reduction here is a different
+//    one (one reduced unknown per chunk instead of two) and the reduced system
+//    is solved in LDS by every lane of the workgroup at once.
+//
+//    A system of length N is cut into P chunks of M rows; lane (w, p) owns rows
+//    s = p*M .. e = s+M-1 of system w in registers (rows >= N are identity rows
+//    a=0, b=1, c=0, d=0; a[0] and c[N-1] are forced to 0).
+//    (i) forward sweep with a left spike: every row i of the chunk becomes
+//          al_i x_{s-1} + x_i + ga_i x_{i+1} = de_i
+//        with inv = 1/(b_i - a_i ga_{i-1}), al_i = -a_i al_{i-1} inv,
+//        ga_i = c_i inv, de_i = (d_i - a_i de_{i-1}) inv (row s: divide by b_s);
+//    (ii) backward sweep expressing rows s..e-1 through x_{s-1} and x_e:
+//          x_i = dp_i - ap_i x_{s-1} - gp_i x_e,
+//        dp_i = de_i - ga_i dp_{i+1}, ap_i = al_i - ga_i ap_{i+1},
+//        gp_i = -ga_i gp_{i+1}, starting from (dp, ap, gp)_e = (0, 0, -1);
+//    (iii) substituting chunk k+1's first row into chunk k's last row gives one
+//        equation per chunk in y_k = x_{e_k}:
+//          al_e y_{k-1} + (1 - ga_e ap'_s) y_k - ga_e gp'_s y_{k+1}
+//                                                   = de_e - ga_e dp'_s
+//        (primes: chunk k+1's first row), a P-row tridiagonal system solved
+//        by parallel cyclic reduction in LDS, ceil(log2 P) steps;
+//    (iv) x_i = dp_i - ap_i y_{k-1} - gp_i y_k inside the chunk, x_e = y_k.
+//    Contiguous systems (solvedim 0, ragged) are loaded coalesced (lane p reads
+//    element p + P*r) and transposed through a skewed LDS tile so that lane p
+//    owns rows p*M..p*M+M-1; the result goes back through the same tile.
+//    Strided systems are loaded directly: W x-neighbouring systems per
+//    workgroup, W consecutive lanes read W consecutive addresses.
+//
+// Division is IEEE (v_div_* sequence) as in the reference CPU code.
+
+#include <algorithm>
+
+#include "trid_internal.h"
+
+namespace trid {
+namespace {
+
+// ---------------------------------------------------------------------------
+// 1. Per-lane Thomas
+// ---------------------------------------------------------------------------
+constexpr int LANE_BLOCK = 256;
+constexpr int LANE_K = 8;  // rows fetched per batch of loads
+
+template <typename REAL, bool INC>
+__global__ __launch_bounds__(LANE_BLOCK) void lane_kernel(
+    Geom g, const REAL *__restrict__ a, const REAL *__restrict__ b,
+    const REAL *__restrict__ c, REAL *__restrict__ d, REAL *__restrict__ u,
+    REAL *__restrict__ cws, REAL *__restrict__ dws) {
+  const int64_t ns = g.nsys();
+  const int64_t s = (int64_t)blockIdx.x * LANE_BLOCK + threadIdx.x;
+  if (s >= ns) return;
+  const int64_t base = (s % g.n0) * g.st0 + (s / g.n0) * g.st1;
+  const int64_t N = g.N, str = g.stride;
+  a += base; b += base; c += base; d += base;
+  if (INC) u += base;
+  // d' lives in d for the plain solve (d is overwritten anyway) and in the
+  // second workspace for Inc, where d must stay untouched.
+  auto dprime = [&](int64_t r) -> REAL & {
+    return INC ? dws[r * ns + s] : d[r * str];
+  };
+
+  REAL bb = REAL(1) / b[0];
+  REAL cc = bb * c[0];
+  REAL dd = bb * d[0];
+  cws[s] = cc;
+  dprime(0) = dd;
+  int64_t r = 1;
+  for (; r + LANE_K <= N; r += LANE_K) {
+    REAL av[LANE_K], bv[LANE_K], cv[LANE_K], dv[LANE_K];
+#pragma unroll
+    for (int k = 0; k < LANE_K; ++k) {
+      const int64_t o = (r + k) * str;
+      av[k] = a[o]; bv[k] = b[o]; cv[k] = c[o]; dv[k] = d[o];
+    }
+#pragma unroll
+    for (int k = 0; k < LANE_K; ++k) {
+      bb = REAL(1) / (bv[k] - av[k] * cc);
+      cc = bb * cv[k];
+      dd = bb * (dv[k] - av[k] * dd);
+      cws[(r + k) * ns + s] = cc;
+      dprime(r + k) = dd;
+    }
+  }
+  for (; r < N; ++r) {
+    const int64_t o = r * str;
+    const REAL aa = a[o];
+    bb = REAL(1) / (b[o] - aa * cc);
+    cc = bb * c[o];
+    dd = bb * (d[o] - aa * dd);
+    cws[r * ns + s] = cc;
+    dprime(r) = dd;
+  }
+
+  // backward: x_{N-1} = d'_{N-1}; x_r = d'_r - c'_r x_{r+1}
+  auto put = [&](int64_t row, REAL x) {
+    if (INC) u[row * str] += x;
+    else d[row * str] = x;
+  };
+  put(N - 1, dd);
+  r = N - 2;
+  for (; r - LANE_K + 1 >= 0; r -= LANE_K) {
+    REAL cv[LANE_K], dv[LANE_K];
+#pragma unroll
+    for (int k = 0; k < LANE_K; ++k) {
+      cv[k] = cws[(r - k) * ns + s];
+      dv[k] = dprime(r - k);
+    }
+#pragma unroll
+    for (int k = 0; k < LANE_K; ++k) {
+      dd = dv[k] - cv[k] * dd;
+      put(r - k, dd);
+    }
+  }
+  for (; r >= 0; --r) {
+    dd = dprime(r) - cws[r * ns + s] * dd;
+    put(r, dd);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 2. Thomas-PCR hybrid
+// ---------------------------------------------------------------------------
+template <typename REAL, int M>
+struct HybridBounds {
+  static constexpr int threads = sizeof(REAL) * M > 64 ? 512 : 1024;
+};
+template <typename REAL, int M>
+constexpr int hybrid_max_threads() {
+  return HybridBounds<REAL, M>::threads;
+}
+
+// Skewed LDS tile index of element e of one system (one pad slot per M
+// elements) so that both the coalesced order (e = p + P*r) and the chunk order
+// (e = p*M + r) are spread over the banks.
+template <int M>
+__device__ __forceinline__ int skew(int e) {
+  return e + e / M;
+}
+
+template <typename REAL, int M, bool INC, bool CONTIG>
+__global__ __launch_bounds__((HybridBounds<REAL, M>::threads)) void hybrid_kernel(
+    Geom g, int P, int W, const REAL *__restrict__ a, const REAL *__restrict__ b,
+    const REAL *__restrict__ c, REAL *__restrict__ d, REAL *__restrict__ u) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  REAL *smem = reinterpret_cast<REAL *>(smem_raw);
+  const int T = W * P;
+  const int t = threadIdx.x;
+  const int p = CONTIG ? t % P : t / W;
+  const int w = CONTIG ? t / P : t % W;
+  const int pstep = CONTIG ? 1 : W;  // slot distance between chunk p and p+1
+
+  const int64_t nb0 = (g.n0 + W - 1) / W;
+  const int64_t i0 = (int64_t)(blockIdx.x % nb0) * W + w;
+  const int64_t i1 = (int64_t)(blockIdx.x / nb0);
+  const bool active = i0 < g.n0;
+  const int64_t base = i0 * g.st0 + i1 * g.st1;
+  const int64_t N = g.N;
+  const int SK = P * (M + 1);  // skewed tile length of one system (contig)
+  REAL *tile = smem + (int64_t)w * SK;
+  REAL *pcr = smem + (CONTIG ? (int64_t)W * SK : 0);  // 2 buffers x 3 x T
+
+  REAL ra[M], rb[M], rc[M], rd[M];
+  if (CONTIG) {
+    REAL va[M], vb[M], vc[M], vd[M];
+#pragma unroll
+    for (int r = 0; r < M; ++r) {
+      const int64_t e = p + (int64_t)P * r;
+      const bool ok = active && e < N;
+      const int64_t o = base + e;
+      va[r] = (ok && e > 0) ? a[o] : REAL(0);
+      vb[r] = ok ? b[o] : REAL(1);
+      vc[r] = (ok && e < N - 1) ? c[o] : REAL(0);
+      vd[r] = ok ? d[o] : REAL(0);
+    }
+    auto transpose_in = [&](const REAL(&v)[M], REAL(&dst)[M]) {
+#pragma unroll
+      for (int r = 0; r < M; ++r) tile[skew<M>(p + P * r)] = v[r];
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < M; ++r) dst[r] = tile[p * (M + 1) + r];
+      __syncthreads();
+    };
+    transpose_in(va, ra);
+    transpose_in(vb, rb);
+    transpose_in(vc, rc);
+    transpose_in(vd, rd);
+  } else {
+#pragma unroll
+    for (int r = 0; r < M; ++r) {
+      const int64_t row = (int64_t)p * M + r;
+      const bool ok = active && row < N;
+      const int64_t o = base + row * g.stride;
+      ra[r] = (ok && row > 0) ? a[o] : REAL(0);
+      rb[r] = ok ? b[o] : REAL(1);
+      rc[r] = (ok && row < N - 1) ? c[o] : REAL(0);
+      rd[r] = ok ? d[o] : REAL(0);
+    }
+  }
+
+  // (i) forward sweep with left spike; al/ga/de overwrite ra/rc/rd.
+  {
+    REAL inv = REAL(1) / rb[0];
+    ra[0] = ra[0] * inv;
+    rc[0] = rc[0] * inv;
+    rd[0] = rd[0] * inv;
+#pragma unroll
+    for (int r = 1; r < M; ++r) {
+      const REAL ar = ra[r];
+      inv = REAL(1) / (rb[r] - ar * rc[r - 1]);
+      ra[r] = -ar * ra[r - 1] * inv;
+      rc[r] = rc[r] * inv;
+      rd[r] = (rd[r] - ar * rd[r - 1]) * inv;
+    }
+  }
+  // Forward values of the chunk's last row, used by the reduced equation.
+  const REAL al_e = ra[M - 1], ga_e = rc[M - 1], de_e = rd[M - 1];
+
+  // (ii) backward sweep over rows 0..M-2: (rd, ra, rc) <- (dp, ap, gp).
+  // Row M-2 keeps its forward values (dp, ap, gp of row e are (0, 0, -1)).
+#pragma unroll
+  for (int r = M - 3; r >= 0; --r) {
+    const REAL gr = rc[r];
+    rd[r] = rd[r] - gr * rd[r + 1];
+    ra[r] = ra[r] - gr * ra[r + 1];
+    rc[r] = -gr * rc[r + 1];
+  }
+  REAL f_d, f_a, f_g;  // first row of this chunk in reduced form
+  if (M >= 2) {
+    f_d = rd[0]; f_a = ra[0]; f_g = rc[0];
+  } else {
+    f_d = REAL(0); f_a = REAL(0); f_g = REAL(-1);
+  }
+
+  // (iii) reduced equation of chunk p; needs chunk p+1's first row.
+  REAL *q0 = pcr, *q1 = pcr + 3 * T;
+  q0[t] = f_d; q0[T + t] = f_a; q0[2 * T + t] = f_g;
+  __syncthreads();
+  REAL nd = 0, na = 0, ng = 0;
+  if (p + 1 < P) {
+    const int sl = t + pstep;
+    nd = q0[sl]; na = q0[T + sl]; ng = q0[2 * T + sl];
+  }
+  REAL A = al_e, Cc = -ga_e * ng, D = de_e - ga_e * nd;
+  {
+    const REAL inv = REAL(1) / (REAL(1) - ga_e * na);
+    A *= inv; Cc *= inv; D *= inv;
+  }
+  // PCR: after the step of distance h, rows couple at distance 2h.
+  REAL *Q = q1;
+  for (int h = 1; h < P; h <<= 1) {
+    Q[t] = A; Q[T + t] = Cc; Q[2 * T + t] = D;
+    __syncthreads();
+    REAL Am = 0, Cm = 0, Dm = 0, Ap = 0, Cp = 0, Dp = 0;
+    if (p - h >= 0) {
+      const int sl = t - h * pstep;
+      Am = Q[sl]; Cm = Q[T + sl]; Dm = Q[2 * T + sl];
+    }
+    if (p + h < P) {
+      const int sl = t + h * pstep;
+      Ap = Q[sl]; Cp = Q[T + sl]; Dp = Q[2 * T + sl];
+    }
+    const REAL inv = REAL(1) / (REAL(1) - A * Cm - Cc * Ap);
+    const REAL nA = -inv * A * Am;
+    const REAL nC = -inv * Cc * Cp;
+    const REAL nD = inv * (D - A * Dm - Cc * Dp);
+    A = nA; Cc = nC; D = nD;
+    Q = (Q == q1) ? q0 : q1;
+  }
+  const REAL y = D;  // x at the last row of chunk p
+  Q[t] = y;
+  __syncthreads();
+  const REAL ym = (p > 0) ? Q[t - pstep] : REAL(0);
+
+  // (iv) interior rows
+  REAL x[M];
+#pragma unroll
+  for (int r = 0; r < M - 1; ++r) x[r] = rd[r] - ra[r] * ym - rc[r] * y;
+  x[M - 1] = y;
+
+  if (CONTIG) {
+    // Q was last written before the barrier above; the tile region is
+    // disjoint from the PCR buffers, and every lane finished reading the tile
+    // before the reduced system started.
+#pragma unroll
+    for (int r = 0; r < M; ++r) tile[p * (M + 1) + r] = x[r];
+    __syncthreads();
+    if (active) {
+#pragma unroll
+      for (int r = 0; r < M; ++r) {
+        const int64_t e = p + (int64_t)P * r;
+        if (e < N) {
+          const REAL v = tile[skew<M>((int)e)];
+          if (INC) u[base + e] += v;
+          else d[base + e] = v;
+        }
+      }
+    }
+  } else if (active) {
+#pragma unroll
+    for (int r = 0; r < M; ++r) {
+      const int64_t row = (int64_t)p * M + r;
+      if (row < N) {
+        const int64_t o = base + row * g.stride;
+        if (INC) u[o] += x[r];
+        else d[o] = x[r];
+      }
+    }
+  }
+}
+
+template <typename REAL, int M, bool INC, bool CONTIG>
+hipError_t launch_hybrid_m(const Geom &g, const HybridCfg &cfg, const REAL *a,
+                           const REAL *b, const REAL *c, REAL *d, REAL *u,
+                           hipStream_t st) {
+  const int T = cfg.W * cfg.P;
+  size_t lds = sizeof(REAL) * (size_t)6 * T;
+  if (CONTIG) lds += sizeof(REAL) * (size_t)cfg.W * cfg.P * (M + 1);
+  auto kern = hybrid_kernel<REAL, M, INC, CONTIG>;
+  if (lds > 65536) {
+    hipError_t e = hipFuncSetAttribute(
+        reinterpret_cast<const void *>(kern),
+        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  const int64_t nb0 = (g.n0 + cfg.W - 1) / cfg.W;
+  const int64_t blocks = nb0 * g.n1;
+  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(T), lds, st, g, cfg.P,
+                     cfg.W, a, b, c, d, u);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+template <typename REAL, bool INC>
+hipError_t launch_lane(const Geom &g, const REAL *a, const REAL *b, const REAL *c,
+                       REAL *d, REAL *u, REAL *cws, REAL *dws, hipStream_t st) {
+  const int64_t ns = g.nsys();
+  const int64_t blocks = (ns + LANE_BLOCK - 1) / LANE_BLOCK;
+  hipLaunchKernelGGL((lane_kernel<REAL, INC>), dim3((unsigned)blocks),
+                     dim3(LANE_BLOCK), 0, st, g, a, b, c, d, u, cws, dws);
+  return hipGetLastError();
+}
+
+template <typename REAL>
+static int max_threads_for_m(int M) {
+  switch (M) {
+    case 1: return hybrid_max_threads<REAL, 1>();
+    case 2: return hybrid_max_threads<REAL, 2>();
+    case 4: return hybrid_max_threads<REAL, 4>();
+    case 8: return hybrid_max_threads<REAL, 8>();
+    case 16: return hybrid_max_threads<REAL, 16>();
+    default: return 0;
+  }
+}
+
+template <typename REAL>
+bool hybrid_cfg_valid(const Geom &g, const HybridCfg &cfg) {
+  const int maxT = max_threads_for_m<REAL>(cfg.M);
+  if (maxT == 0 || cfg.P < 1 || cfg.W < 1) return false;
+  if ((int64_t)cfg.P * cfg.M < g.N) return false;
+  const int64_t T = (int64_t)cfg.W * cfg.P;
+  if (T > maxT) return false;
+  size_t lds = sizeof(REAL) * (size_t)6 * T;
+  if (g.stride == 1) lds += sizeof(REAL) * (size_t)T * (cfg.M + 1);
+  return lds <= 160 * 1024;
+}
+
+template <typename REAL>
+bool hybrid_default_cfg(const Geom &g, HybridCfg *cfg) {
+  const bool contig = g.stride == 1;
+  static const int Ms[] = {1, 2, 4, 8, 16};
+  if (contig) {
+    // One wave (or fewer lanes) per system when possible, 256-thread groups.
+    for (int M : Ms) {
+      const int64_t P = (g.N + M - 1) / M;
+      if (P > 64 && M < 16) continue;
+      HybridCfg c;
+      c.M = M;
+      c.P = (int)P;
+      c.W = (int)std::max<int64_t>(1, 256 / P);
+      if (hybrid_cfg_valid<REAL>(g, c)) { *cfg = c; return true; }
+    }
+    return false;
+  }
+  // Strided: 128 bytes of neighbouring systems per row, groups of <= 512.
+  for (int W = 128 / (int)sizeof(REAL); W >= 4; W /= 2) {
+    for (int M : Ms) {
+      HybridCfg c;
+      c.M = M;
+      c.P = (int)((g.N + M - 1) / M);
+      c.W = W;
+      if ((int64_t)c.W * c.P > 512) continue;
+      if (hybrid_cfg_valid<REAL>(g, c)) { *cfg = c; return true; }
+    }
+  }
+  return false;
+}
+
+template <typename REAL, bool INC>
+hipError_t launch_hybrid(const Geom &g, const HybridCfg &cfg, const REAL *a,
+                         const REAL *b, const REAL *c, REAL *d, REAL *u,
+                         hipStream_t st) {
+  const bool contig = g.stride == 1;
+#define TRID_HYB_CASE(MM)                                                        \
+  case MM:                                                                       \
+    return contig ? launch_hybrid_m<REAL, MM, INC, true>(g, cfg, a, b, c, d, u, st) \
+                  : launch_hybrid_m<REAL, MM, INC, false>(g, cfg, a, b, c, d, u, st);
+  switch (cfg.M) {
+    TRID_HYB_CASE(1)
+    TRID_HYB_CASE(2)
+    TRID_HYB_CASE(4)
+    TRID_HYB_CASE(8)
+    TRID_HYB_CASE(16)
+    default:
+      return hipErrorInvalidValue;
+  }
+#undef TRID_HYB_CASE
+}
+
+#define TRID_INST(REAL)                                                          \
+  template hipError_t launch_lane<REAL, false>(const Geom &, const REAL *,       \
+      const REAL *, const REAL *, REAL *, REAL *, REAL *, REAL *, hipStream_t);  \
+  template hipError_t launch_lane<REAL, true>(const Geom &, const REAL *,        \
+      const REAL *, const REAL *, REAL *, REAL *, REAL *, REAL *, hipStream_t);  \
+  template hipError_t launch_hybrid<REAL, false>(const Geom &, const HybridCfg &,\
+      const REAL *, const REAL *, const REAL *, REAL *, REAL *, hipStream_t);    \
+  template hipError_t launch_hybrid<REAL, true>(const Geom &, const HybridCfg &, \
+      const REAL *, const REAL *, const REAL *, REAL *, REAL *, hipStream_t);    \
+  template bool hybrid_cfg_valid<REAL>(const Geom &, const HybridCfg &);         \
+  template bool hybrid_default_cfg<REAL>(const Geom &, HybridCfg *);
+
+TRID_INST(float)
+TRID_INST(double)
+
+}  // namespace trid
