@@ -40,13 +40,14 @@ EXTERN_C size_t tridHipWorkspaceBytes(int fp64, int inc, int ndim, int solvedim,
  * algorithm ids: 1 = per-lane Thomas, 3 = Thomas-PCR hybrid. */
 
 /* What the last solve of this thread ran: algorithm, and for the hybrid its
- * rows per lane M, lanes per system P and systems per workgroup W. */
-EXTERN_C tridStatus_t tridHipLastConfig(int *algorithm, int *M, int *P, int *W);
+ * rows per lane M, lanes per system P, systems per workgroup W and XCD
+ * grouping G (consecutive system groups dealt to one XCD; 1 = plain order). */
+EXTERN_C tridStatus_t tridHipLastConfig(int *algorithm, int *M, int *P, int *W, int *G);
 
 /* Forces the hybrid shape for this thread's subsequent solves (M = 0 clears).
  * A shape that does not fit a call (P*M < length, too many threads or LDS)
  * makes that call return TRID_STATUS_INVALID_VALUE. */
-EXTERN_C tridStatus_t tridHipForceHybridConfig(int M, int P, int W);
+EXTERN_C tridStatus_t tridHipForceHybridConfig(int M, int P, int W, int G);
 
 /* ---- measurement ---------------------------------------------------------
  * No counterpart in the reference. Streams `bytes` (a multiple of 16, both

@@ -120,17 +120,18 @@ def test_random_vs_oracle(dims, dtype, inc, algo):
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("M", [1, 2, 4, 8, 16])
-@pytest.mark.parametrize("W", [1, 3, 16])
-def test_forced_hybrid_shapes(M, W, dtype):
-    """Every compiled M, several W, lengths that are not multiples of M."""
-    for dims, sd in (([61, 5, 7], 0), ([19, 45, 3], 1), ([17, 4, 51], 2)):
+@pytest.mark.parametrize("W,G", [(1, 1), (3, 8), (16, 8), (16, 3)])
+def test_forced_hybrid_shapes(M, W, G, dtype):
+    """Every compiled M, several W and XCD groupings, lengths that are not
+    multiples of M, block counts that are not multiples of G."""
+    for dims, sd in (([61, 5, 7], 0), ([19, 45, 3], 1), ([17, 4, 51], 2), ([40, 37, 9], 1)):
         N = dims[sd]
         P = -(-N // M)
         if W * P > (512 if np.dtype(dtype).itemsize * M > 64 else 1024):
             continue
-        with api.forced_hybrid(M, P, W):
+        with api.forced_hybrid(M, P, W, G):
             cfg = run_case(dims, sd, dtype, False, (0, 0, 0), seed=M * 7 + W)
-        assert cfg == (3, M, P, W)
+        assert cfg == (3, M, P, W, G)
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])

@@ -60,8 +60,8 @@ SIGNATURES = {
     "tridHipReserveWorkspace": ([SZ], I),
     "tridHipReleaseWorkspaces": ([], I),
     "tridHipWorkspaceBytes": ([I, I, I, I, P], SZ),
-    "tridHipLastConfig": ([IP, IP, IP, IP], I),
-    "tridHipForceHybridConfig": ([I, I, I], I),
+    "tridHipLastConfig": ([IP, IP, IP, IP, IP], I),
+    "tridHipForceHybridConfig": ([I, I, I, I], I),
     "tridHipStreamCopy": ([P, P, SZ], I),
 }
 

@@ -91,23 +91,23 @@ def trid_strided_batch(a, b, c, d, *, solvedim, dims=None, pads=None, u=None,
 
 
 def last_config():
-    """(algorithm, M, P, W) of this thread's last solve (1 lane, 3 hybrid)."""
-    v = [C.c_int(0) for _ in range(4)]
+    """(algorithm, M, P, W, G) of this thread's last solve (1 lane, 3 hybrid)."""
+    v = [C.c_int(0) for _ in range(5)]
     _lib.lib().tridHipLastConfig(*[C.byref(x) for x in v])
     return tuple(x.value for x in v)
 
 
 class forced_hybrid:
     """Context manager forcing the hybrid shape (M rows/lane, P lanes/system,
-    W systems/workgroup) for this thread's solves."""
+    W systems/workgroup, G XCD grouping) for this thread's solves."""
 
-    def __init__(self, M, P, W):
-        self.cfg = (M, P, W)
+    def __init__(self, M, P, W, G=8):
+        self.cfg = (M, P, W, G)
 
     def __enter__(self):
         check(_lib.lib().tridHipForceHybridConfig(*self.cfg), "tridHipForceHybridConfig")
         return self
 
     def __exit__(self, *exc):
-        _lib.lib().tridHipForceHybridConfig(0, 0, 0)
+        _lib.lib().tridHipForceHybridConfig(0, 0, 0, 0)
         return False

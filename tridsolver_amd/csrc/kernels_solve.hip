@@ -147,6 +147,17 @@ constexpr int hybrid_max_threads() {
   return HybridBounds<REAL, M>::threads;
 }
 
+// Maps the hardware workgroup id to the system group it solves. Workgroups
+// are handed to the XCDs round-robin (id % 8); dealing ids so that group q of
+// G gets a contiguous range makes neighbouring system groups -- which read
+// neighbouring 128-byte row segments -- run on one XCD at the same time.
+__device__ __forceinline__ int64_t xcd_remap(int64_t bid, int64_t nb, int G) {
+  if (G <= 1) return bid;
+  const int64_t per = nb / G;
+  if (bid >= per * G) return bid;  // tail beyond a multiple of G: identity
+  return (bid % G) * per + bid / G;
+}
+
 // Skewed LDS tile index of element e of one system (one pad slot per M
 // elements) so that both the coalesced order (e = p + P*r) and the chunk order
 // (e = p*M + r) are spread over the banks.
@@ -157,8 +168,9 @@ __device__ __forceinline__ int skew(int e) {
 
 template <typename REAL, int M, bool INC, bool CONTIG>
 __global__ __launch_bounds__((HybridBounds<REAL, M>::threads)) void hybrid_kernel(
-    Geom g, int P, int W, const REAL *__restrict__ a, const REAL *__restrict__ b,
-    const REAL *__restrict__ c, REAL *__restrict__ d, REAL *__restrict__ u) {
+    Geom g, int P, int W, int G, const REAL *__restrict__ a,
+    const REAL *__restrict__ b, const REAL *__restrict__ c, REAL *__restrict__ d,
+    REAL *__restrict__ u) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   REAL *smem = reinterpret_cast<REAL *>(smem_raw);
   const int T = W * P;
@@ -168,8 +180,9 @@ __global__ __launch_bounds__((HybridBounds<REAL, M>::threads)) void hybrid_kerne
   const int pstep = CONTIG ? 1 : W;  // slot distance between chunk p and p+1
 
   const int64_t nb0 = (g.n0 + W - 1) / W;
-  const int64_t i0 = (int64_t)(blockIdx.x % nb0) * W + w;
-  const int64_t i1 = (int64_t)(blockIdx.x / nb0);
+  const int64_t bid = xcd_remap(blockIdx.x, gridDim.x, G);
+  const int64_t i0 = (bid % nb0) * W + w;
+  const int64_t i1 = bid / nb0;
   const bool active = i0 < g.n0;
   const int64_t base = i0 * g.st0 + i1 * g.st1;
   const int64_t N = g.N;
@@ -343,7 +356,7 @@ hipError_t launch_hybrid_m(const Geom &g, const HybridCfg &cfg, const REAL *a,
   const int64_t nb0 = (g.n0 + cfg.W - 1) / cfg.W;
   const int64_t blocks = nb0 * g.n1;
   hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(T), lds, st, g, cfg.P,
-                     cfg.W, a, b, c, d, u);
+                     cfg.W, cfg.G, a, b, c, d, u);
   return hipGetLastError();
 }
 
@@ -403,15 +416,20 @@ bool hybrid_default_cfg(const Geom &g, HybridCfg *cfg) {
     }
     return false;
   }
-  // Strided: 128 bytes of neighbouring systems per row, groups of <= 512.
-  for (int W = 128 / (int)sizeof(REAL); W >= 4; W /= 2) {
-    for (int M : Ms) {
-      HybridCfg c;
-      c.M = M;
-      c.P = (int)((g.N + M - 1) / M);
-      c.W = W;
-      if ((int64_t)c.W * c.P > 512) continue;
-      if (hybrid_cfg_valid<REAL>(g, c)) { *cfg = c; return true; }
+  // Strided: 128 bytes of neighbouring systems per row (W = 16 FP64 / 32
+  // FP32); the smallest M whose workgroup fits the preferred size (FP64: 512
+  // threads at M = 16, measured best at 512^3), else any valid shape.
+  const int pref = sizeof(REAL) == 8 ? 512 : 1024;
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int W = 128 / (int)sizeof(REAL); W >= 4; W /= 2) {
+      for (int M : Ms) {
+        HybridCfg c;
+        c.M = M;
+        c.P = (int)((g.N + M - 1) / M);
+        c.W = W;
+        if (pass == 0 && (int64_t)c.W * c.P > pref) continue;
+        if (hybrid_cfg_valid<REAL>(g, c)) { *cfg = c; return true; }
+      }
     }
   }
   return false;

@@ -26,7 +26,8 @@ namespace trid {
 namespace {
 
 thread_local hipStream_t tl_stream = nullptr;
-thread_local int tl_last_algo = 0, tl_last_m = 0, tl_last_p = 0, tl_last_w = 0;
+thread_local int tl_last_algo = 0, tl_last_m = 0, tl_last_p = 0, tl_last_w = 0,
+                 tl_last_g = 0;
 thread_local HybridCfg tl_forced;
 
 std::mutex g_mu;
@@ -130,6 +131,7 @@ void record_config(int algo, const HybridCfg *cfg) {
   tl_last_m = cfg ? cfg->M : 0;
   tl_last_p = cfg ? cfg->P : 0;
   tl_last_w = cfg ? cfg->W : 0;
+  tl_last_g = cfg ? cfg->G : 0;
 }
 
 const HybridCfg &forced_config() { return tl_forced; }
@@ -259,22 +261,24 @@ EXTERN_C size_t tridHipWorkspaceBytes(int fp64, int inc, int ndim, int solvedim,
   return inc ? 2 * one : one;
 }
 
-EXTERN_C tridStatus_t tridHipLastConfig(int *algorithm, int *M, int *P, int *W) {
+EXTERN_C tridStatus_t tridHipLastConfig(int *algorithm, int *M, int *P, int *W, int *G) {
   if (algorithm) *algorithm = tl_last_algo;
   if (M) *M = tl_last_m;
   if (P) *P = tl_last_p;
   if (W) *W = tl_last_w;
+  if (G) *G = tl_last_g;
   return TRID_STATUS_SUCCESS;
 }
 
-EXTERN_C tridStatus_t tridHipForceHybridConfig(int M, int P, int W) {
+EXTERN_C tridStatus_t tridHipForceHybridConfig(int M, int P, int W, int G) {
   if (M == 0) {
     tl_forced = HybridCfg();
     return TRID_STATUS_SUCCESS;
   }
-  if (M < 0 || P < 1 || W < 1) return TRID_STATUS_INVALID_VALUE;
+  if (M < 0 || P < 1 || W < 1 || G < 1) return TRID_STATUS_INVALID_VALUE;
   tl_forced.M = M;
   tl_forced.P = P;
   tl_forced.W = W;
+  tl_forced.G = G;
   return TRID_STATUS_SUCCESS;
 }

@@ -27,11 +27,15 @@ struct Geom {
 };
 
 // Tuning of the Thomas-PCR hybrid for one call: M rows per lane, P lanes per
-// system, W systems per workgroup (block = W * P threads).
+// system, W systems per workgroup (block = W * P threads), and G, the number
+// of groups the workgroups are dealt into so that consecutive system groups
+// run on the same XCD at the same time (workgroups are dispatched to the 8
+// XCDs round-robin; G = 1 keeps the plain order).
 struct HybridCfg {
   int M = 0;
   int P = 0;
   int W = 0;
+  int G = 8;
 };
 
 enum class Algo : int { Auto = 0, Lane = 1, Hybrid = 3 };
