@@ -63,6 +63,13 @@ SIGNATURES = {
     "tridHipLastConfig": ([IP, IP, IP, IP, IP], I),
     "tridHipForceHybridConfig": ([I, I, I, I], I),
     "tridHipStreamCopy": ([P, P, SZ], I),
+    # include/tridsolver_adi.h
+    "tridAdiPreprocD": ([D] + [P] * 11 + [I, I, I], I),
+    "tridAdiPreprocS": ([F] + [P] * 11 + [I, I, I], I),
+    "tridAdiStepD": ([P, D] + [P] * 5 + [I, I, I], I),
+    "tridAdiStepS": ([P, F] + [P] * 5 + [I, I, I], I),
+    "tridAdiRunD": ([P, D] + [P] * 5 + [I, I, I, I, I], I),
+    "tridAdiRunS": ([P, F] + [P] * 5 + [I, I, I, I, I], I),
 }
 
 _lib = None

@@ -47,6 +47,7 @@
 // Division is IEEE (v_div_* sequence) as in the reference CPU code.
 
 #include <algorithm>
+#include <atomic>
 
 #include "trid_internal.h"
 
@@ -347,11 +348,15 @@ hipError_t launch_hybrid_m(const Geom &g, const HybridCfg &cfg, const REAL *a,
   size_t lds = sizeof(REAL) * (size_t)6 * T;
   if (CONTIG) lds += sizeof(REAL) * (size_t)cfg.W * cfg.P * (M + 1);
   auto kern = hybrid_kernel<REAL, M, INC, CONTIG>;
-  if (lds > 65536) {
+  // Raise the dynamic LDS limit once per kernel (not capturable, so it is not
+  // repeated inside a hipGraph capture of a later call).
+  static std::atomic<int> lds_raised{0};
+  if (lds > 65536 && !lds_raised.load()) {
     hipError_t e = hipFuncSetAttribute(
         reinterpret_cast<const void *>(kern),
-        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     if (e != hipSuccess) return e;
+    lds_raised.store(1);
   }
   const int64_t nb0 = (g.n0 + cfg.W - 1) / cfg.W;
   const int64_t blocks = nb0 * g.n1;
