@@ -20,6 +20,10 @@
  * calling thread is enqueued on the thread's library stream (default: the
  * null stream); ctx->sync decides whether the call waits for it. */
 
+/* Text of the HIP error behind this thread's last TRID_STATUS_EXECUTION_FAILED
+ * ("no error" if none); a static string. */
+EXTERN_C const char *tridHipLastErrorString(void);
+
 /* Sets the stream used by this thread's subsequent calls (NULL = null stream). */
 EXTERN_C tridStatus_t tridHipSetStream(void *stream);
 EXTERN_C tridStatus_t tridHipGetStream(void **stream);
@@ -37,9 +41,10 @@ EXTERN_C size_t tridHipWorkspaceBytes(int fp64, int inc, int ndim, int solvedim,
                                       const int *dims);
 
 /* ---- introspection and tuning --------------------------------------------
- * algorithm ids: 1 = per-lane Thomas, 3 = Thomas-PCR hybrid. */
+ */
 
-/* What the last solve of this thread ran: algorithm, and for the hybrid its
+/* What the last solve of this thread ran: algorithm (1 per-lane Thomas,
+ * 2 mixed precision, 3 hybrid, 4 adjoint, 5 ragged), and for the hybrid its
  * rows per lane M, lanes per system P, systems per workgroup W and XCD
  * grouping G (consecutive system groups dealt to one XCD; 1 = plain order). */
 EXTERN_C tridStatus_t tridHipLastConfig(int *algorithm, int *M, int *P, int *W, int *G);
@@ -48,6 +53,74 @@ EXTERN_C tridStatus_t tridHipLastConfig(int *algorithm, int *M, int *P, int *W, 
  * A shape that does not fit a call (P*M < length, too many threads or LDS)
  * makes that call return TRID_STATUS_INVALID_VALUE. */
 EXTERN_C tridStatus_t tridHipForceHybridConfig(int M, int P, int W, int G);
+
+/* ---- FP32 sweeps with FP64 refinement ------------------------------------
+ * No counterpart in the reference. Same arguments and layout as
+ * tridDmtsvStridedBatch (scalar/include/tridsolver.h:38-42) plus the number
+ * of refinement steps (0..8; 2 reaches FP64 accuracy on diagonally dominant
+ * systems). The matrix is factored and the sweeps run in FP32; the residual
+ * d - A x and the accumulation x += dx are FP64. One pass over HBM: a, b, c,
+ * d are read once and d is overwritten with the FP64 solution. Systems up to
+ * 4096 rows (8 rows per lane, 512 lanes). */
+EXTERN_C tridStatus_t tridDmtsvStridedBatchMixed(const TridParams *ctx, const double *a,
+                                                 const double *b, const double *c, double *d,
+                                                 int ndim, int solvedim, const int *dims,
+                                                 const int *pads, int refinements);
+
+/* ---- adjoint ---------------------------------------------------------------
+ * No counterpart in the reference. For the solve x = A^-1 d along `solvedim`
+ * (same layout arguments as tridDmtsvStridedBatch) and w = dL/dx, solves
+ * A^T lam = w and writes lam (= dL/dd) and, when the pointers are not NULL,
+ * dL/da = -lam * x[n-1], dL/db = -lam * x[n], dL/dc = -lam * x[n+1]; the
+ * entries a[0] and c[N-1] of every system get 0 (they are unused by the
+ * solve). x must be the forward solution. Systems up to one workgroup long
+ * (16384 rows FP32, 8192 FP64 on contiguous systems). */
+EXTERN_C tridStatus_t tridDmtsvStridedBatchAdjoint(const TridParams *ctx, const double *a,
+                                                   const double *b, const double *c,
+                                                   const double *x, const double *w,
+                                                   double *lam, double *grad_a,
+                                                   double *grad_b, double *grad_c, int ndim,
+                                                   int solvedim, const int *dims,
+                                                   const int *pads);
+EXTERN_C tridStatus_t tridSmtsvStridedBatchAdjoint(const TridParams *ctx, const float *a,
+                                                   const float *b, const float *c,
+                                                   const float *x, const float *w, float *lam,
+                                                   float *grad_a, float *grad_b, float *grad_c,
+                                                   int ndim, int solvedim, const int *dims,
+                                                   const int *pads);
+
+/* ---- ragged batches --------------------------------------------------------
+ * No counterpart in the reference (its API needs one length for all systems).
+ * nsys independent systems stored back to back: system s occupies elements
+ * offsets[s] .. offsets[s+1]-1 of a, b, c, d (offsets: nsys+1 int64 values in
+ * device memory, non-decreasing). Within a system the data contract of
+ * tridsolver.h holds (a[first] and c[last] unused). max_len is an upper bound
+ * of every length, known to the caller (at most 8192, or 4096 with FP32); a
+ * longer system found on the device is skipped and, for a synchronous call,
+ * reported as TRID_STATUS_INVALID_VALUE. One persistent launch after four
+ * small planning kernels; capturable. Needs tridHipRaggedWorkspaceBytes(nsys)
+ * of the stream's workspace (reserve it before a hipGraph capture). */
+EXTERN_C tridStatus_t tridDmtsvRaggedBatch(const TridParams *ctx, const double *a,
+                                           const double *b, const double *c, double *d,
+                                           const int64_t *offsets, int64_t nsys,
+                                           int64_t max_len);
+EXTERN_C tridStatus_t tridSmtsvRaggedBatch(const TridParams *ctx, const float *a,
+                                           const float *b, const float *c, float *d,
+                                           const int64_t *offsets, int64_t nsys,
+                                           int64_t max_len);
+/* Same, adding the solutions into u (d is read only). */
+EXTERN_C tridStatus_t tridDmtsvRaggedBatchInc(const TridParams *ctx, const double *a,
+                                              const double *b, const double *c, double *d,
+                                              double *u, const int64_t *offsets, int64_t nsys,
+                                              int64_t max_len);
+EXTERN_C tridStatus_t tridSmtsvRaggedBatchInc(const TridParams *ctx, const float *a,
+                                              const float *b, const float *c, float *d,
+                                              float *u, const int64_t *offsets, int64_t nsys,
+                                              int64_t max_len);
+EXTERN_C size_t tridHipRaggedWorkspaceBytes(int64_t nsys);
+/* Forces the ragged kernel's rows per lane (4, 8, or 16 at FP64) and
+ * workgroups per CU for this thread's calls; M = 0 restores the tuned choice. */
+EXTERN_C tridStatus_t tridHipForceRaggedConfig(int M, int per_cu);
 
 /* ---- measurement ---------------------------------------------------------
  * No counterpart in the reference. Streams `bytes` (a multiple of 16, both

@@ -1,0 +1,321 @@
+"""GPU tests of the extensions: FP32 sweeps + FP64 refinement, the adjoint
+(and its autograd wrapper), and ragged batches in one persistent launch.
+
+Tolerances (normwise, max|err| / max|ref|):
+  mixed, 2 refinements, vs the long-double oracle  <= 1e-12
+  mixed, 0 refinements (pure FP32 sweeps)          <= 1e-5
+  adjoint FP64 vs the oracle's FP64 adjoint        <= 1e-12 (each output)
+  adjoint FP32 vs the same                          <= 1e-5
+  central differences, h = 1e-6, vs the adjoint     <= 1e-6 absolute + relative
+  ragged vs the oracle per system                   <= 1e-12 FP64, 1e-5 FP32
+"""
+import numpy as np
+import pytest
+
+import oracle
+from tests import helpers as H
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+from tridsolver_amd import api  # noqa: E402
+from tridsolver_amd.autograd import trid_solve  # noqa: E402
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    torch.cuda.set_device(0)
+
+
+def dev(x):
+    return torch.from_numpy(np.ascontiguousarray(x)).cuda()
+
+
+# --------------------------------------------------------------------------
+# FP32 sweeps + FP64 refinement
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("dims", [[64, 48, 40], [37, 21, 19], [512, 3, 2], [9, 700, 2],
+                                  [1000]])
+def test_mixed_refinement_accuracy(dims):
+    rng = np.random.default_rng(sum(dims))
+    n = int(np.prod(dims))
+    a, b, c, d = H.diag_dominant(rng, n)
+    for sd in range(len(dims)):
+        xl = oracle.batch_ld(a, b, c, d, ndim=len(dims), solvedim=sd, dims=dims)
+        errs = []
+        for refine in (0, 1, 2):
+            td = dev(d)
+            api.mixed_strided_batch(dev(a), dev(b), dev(c), td, solvedim=sd, dims=dims,
+                                    refinements=refine, sync=True)
+            errs.append(H.norm_rel_err(td.cpu().numpy(), xl))
+        assert errs[0] <= 1e-5, errs
+        assert errs[2] <= 1e-12, errs
+        assert errs[1] < errs[0]
+        assert api.last_config()[0] == 2
+
+
+def test_mixed_padded_layout():
+    rng = np.random.default_rng(77)
+    dims = [30, 22, 14]
+    n = int(np.prod(dims))
+    a, b, c, d = H.diag_dominant(rng, n)
+    arrs = [H.pad_halo(t, dims) for t in (a, b, c, d)]
+    off, pads = H.halo_offset(dims), [t + 2 for t in dims]
+    for sd in range(3):
+        td = dev(arrs[3])
+        api.mixed_strided_batch(*(dev(t) for t in arrs[:3]), td, solvedim=sd, dims=dims,
+                                pads=pads, offset=off, refinements=2, sync=True)
+        od = arrs[3].copy()
+        oracle.batch(arrs[0], arrs[1], arrs[2], od, ndim=3, solvedim=sd, dims=dims, pads=pads,
+                     offset=off)
+        got = td.cpu().numpy()
+        H.assert_norm_close(got, od, np.float64)
+        mask = H.pad_halo(np.ones(n, dtype=bool), dims)
+        assert np.array_equal(got[~mask], arrs[3][~mask])
+
+
+# --------------------------------------------------------------------------
+# Adjoint
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("dims", [[40, 36, 28], [37, 21, 19], [700, 3], [5, 300]])
+def test_adjoint_vs_oracle(dims, dtype):
+    rng = np.random.default_rng(len(dims) * 31 + dims[0])
+    n = int(np.prod(dims))
+    a, b, c, d = H.diag_dominant(rng, n)
+    w = rng.standard_normal(n)
+    tol = 1e-12 if dtype == np.float64 else 1e-5
+    for sd in range(len(dims)):
+        x = d.copy()
+        oracle.batch(a, b, c, x, ndim=len(dims), solvedim=sd, dims=dims)
+        ref = oracle.adjoint(a, b, c, x, w, ndim=len(dims), solvedim=sd, dims=dims)
+        t = [dev(v.astype(dtype)) for v in (a, b, c, x, w)]
+        out = api.adjoint_strided_batch(*t, solvedim=sd, dims=dims, grad_a=True, grad_b=True,
+                                        grad_c=True, sync=True)
+        for got, want in zip(out, ref):
+            H.assert_norm_close(got.cpu().numpy(), want, tol=tol)
+        # only lam requested: gradients untouched
+        lam_only = api.adjoint_strided_batch(*t, solvedim=sd, dims=dims, sync=True)
+        assert lam_only[1] is None and lam_only[2] is None and lam_only[3] is None
+        H.assert_norm_close(lam_only[0].cpu().numpy(), ref[0], tol=tol)
+        # duality <w, A^-1 d> = <A^-T w, d>
+        lam = out[0].cpu().numpy().astype(np.float64)
+        assert abs(np.dot(w, x) - np.dot(lam, d)) <= 50 * tol * np.abs(w).sum()
+
+
+def _fd_check(a, b, c, d, w, dims, sd, grads, rng, count=100, h=1e-6):
+    """Central differences of L = sum(w * A^-1 d) on `count` random entries of
+    each of a, b, c, d, evaluated on the GPU; L is summed over the perturbed
+    entry's own system only (the other systems do not depend on it)."""
+    names = "abcd"
+    arrs = [a, b, c, d]
+    n = len(d)
+    sys_of = np.zeros(n, dtype=np.int64)
+    slices = list(H.system_slices(len(dims), sd, dims))
+    for k, idx in enumerate(slices):
+        sys_of[idx] = k
+    td = [dev(t) for t in arrs]
+    tw = dev(w)
+    picks = {k: rng.choice(n, count, replace=False) for k in range(4)}
+    worst = 0.0
+    for k in range(4):
+        for i in picks[k]:
+            idx = torch.from_numpy(slices[sys_of[i]]).cuda()
+            vals = []
+            for sgn in (1, -1):
+                t = td[k].clone()
+                t[i] += sgn * h
+                args = [td[j] if j != k else t for j in range(4)]
+                x = args[3].clone()
+                api.trid_strided_batch(args[0], args[1], args[2], x, solvedim=sd, dims=dims)
+                vals.append(float((tw[idx] * x[idx]).sum()))
+            fd = (vals[0] - vals[1]) / (2 * h)
+            g = float(grads[k][i])
+            err = abs(fd - g) / max(1.0, abs(g))
+            worst = max(worst, err)
+            assert err <= 1e-6, (names[k], int(i), fd, g)
+    return worst
+
+
+@pytest.mark.parametrize("sd", [0, 1, 2])
+def test_adjoint_central_differences(sd):
+    rng = np.random.default_rng(100 + sd)
+    dims = [24, 20, 18]
+    n = int(np.prod(dims))
+    a, b, c, d = H.diag_dominant(rng, n)
+    w = rng.standard_normal(n)
+    x = d.copy()
+    oracle.batch(a, b, c, x, ndim=3, solvedim=sd, dims=dims)
+    lam, ga, gb, gc = (t.cpu().numpy() for t in api.adjoint_strided_batch(
+        dev(a), dev(b), dev(c), dev(x), dev(w), solvedim=sd, dims=dims, grad_a=True,
+        grad_b=True, grad_c=True, sync=True))
+    _fd_check(a, b, c, d, w, dims, sd, [ga, gb, gc, lam], rng)
+
+
+@pytest.mark.parametrize("sd", [0, 1, 2])
+def test_config_adjoint_256_fp64(sd):
+    """Differentiable solve on 256^3 FP64: forward x = A^-1 d, L = sum(w x),
+    w ~ N(0,1); adjoint gradients wrt a, b, c, d vs the CPU FP64 adjoint and
+    vs central differences (h = 1e-6) on 100 random entries of each."""
+    rng = np.random.default_rng(256 + sd)
+    dims = [256, 256, 256]
+    n = 256 ** 3
+    a, b, c, d = H.diag_dominant(rng, n)
+    w = rng.standard_normal(n)
+    ta, tb, tc, td = (dev(t) for t in (a, b, c, d))
+    tx = td.clone()
+    api.trid_strided_batch(ta, tb, tc, tx, solvedim=sd, dims=dims, sync=True)
+    x = tx.cpu().numpy()
+    out = api.adjoint_strided_batch(ta, tb, tc, tx, dev(w), solvedim=sd, dims=dims,
+                                    grad_a=True, grad_b=True, grad_c=True, sync=True)
+    lam, ga, gb, gc = (t.cpu().numpy() for t in out)
+    ref = oracle.adjoint(a, b, c, x, w, ndim=3, solvedim=sd, dims=dims)
+    errs = [H.assert_norm_close(g, r, tol=1e-12) for g, r in zip((lam, ga, gb, gc), ref)]
+    worst = _fd_check(a, b, c, d, w, dims, sd, [ga, gb, gc, lam], rng)
+    print("adjoint 256^3 sd=%d: vs CPU adjoint %s, worst FD rel err %.2e"
+          % (sd, ["%.1e" % e for e in errs], worst))
+
+
+def test_autograd_gradcheck_and_dense_reference():
+    rng = np.random.default_rng(3)
+    shape = (5, 4, 6)  # torch order [z, y, x]
+    n = int(np.prod(shape))
+    a, b, c, d = (dev(t).reshape(shape).requires_grad_() for t in H.diag_dominant(rng, n))
+    for sd in range(3):
+        assert torch.autograd.gradcheck(lambda *t: trid_solve(*t, sd), (a, b, c, d),
+                                        eps=1e-6, atol=1e-8, rtol=1e-6)
+    # dense reference on the CPU for solvedim 0 (last torch dim)
+    x = trid_solve(a, b, c, d, 0)
+    wt = torch.randn_like(x)
+    (x * wt).sum().backward()
+    A = torch.diag_embed(b.detach().cpu()) + torch.diag_embed(a.detach().cpu()[..., 1:], -1) + \
+        torch.diag_embed(c.detach().cpu()[..., :-1], 1)
+    Ad = A.clone().requires_grad_()
+    dd = d.detach().cpu().clone().requires_grad_()
+    xd = torch.linalg.solve(Ad, dd.unsqueeze(-1)).squeeze(-1)
+    (xd * wt.cpu()).sum().backward()
+    np.testing.assert_allclose(d.grad.cpu().numpy(), dd.grad.numpy(), rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(b.grad.cpu().numpy(),
+                               torch.diagonal(Ad.grad, dim1=-2, dim2=-1).numpy(), rtol=1e-12,
+                               atol=1e-14)
+    ga = a.grad.cpu().numpy()
+    assert np.all(ga[..., 0] == 0)
+    np.testing.assert_allclose(ga[..., 1:], torch.diagonal(Ad.grad, -1, -2, -1).numpy(),
+                               rtol=1e-12, atol=1e-14)
+
+
+# --------------------------------------------------------------------------
+# Ragged batches
+# --------------------------------------------------------------------------
+def ragged_case(rng, lens, dtype):
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    a, b, c, d = H.diag_dominant(rng, int(off[-1]), dtype)
+    return off, a, b, c, d
+
+
+def check_ragged(off, a, b, c, d, got, dtype, u0=None):
+    want = d.copy()
+    u = None if u0 is None else u0.copy()
+    oracle.ragged(a, b, c, want, off, u, inc=u0 is not None)
+    ref = want if u0 is None else u
+    tol = 1e-12 if dtype == np.float64 else 1e-5
+    if u0 is None:
+        H.assert_norm_close(got, ref, dtype, tol=tol)
+    else:
+        H.assert_norm_close(got - u0, ref - u0, dtype, tol=tol)
+    # per-system check on a sample
+    for s in np.random.default_rng(1).choice(len(off) - 1, min(50, len(off) - 1), False):
+        sl = slice(off[s], off[s + 1])
+        if off[s + 1] > off[s]:
+            x, r = got[sl].astype(np.float64), ref[sl].astype(np.float64)
+            assert np.abs(x - r).max() <= tol * max(np.abs(r).max(), 1e-30) * 10
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("M", [0, 4, 8, 16])
+def test_ragged_vs_oracle(dtype, M):
+    if M == 16 and dtype == np.float32:
+        pytest.skip("FP32 ragged kernel is built for M = 4, 8")
+    rng = np.random.default_rng(M + 1)
+    max_len = 2048 if M == 4 and dtype == np.float64 else 4096
+    lens = np.round(np.exp(rng.uniform(np.log(1), np.log(max_len), 3000))).astype(np.int64)
+    lens[:5] = [0, 1, 2, 0, max_len]
+    off, a, b, c, d = ragged_case(rng, lens, dtype)
+    ctx = api.forced_ragged(M) if M else None
+    if ctx:
+        ctx.__enter__()
+    try:
+        for inc in (False, True):
+            td = dev(d)
+            tu = dev(rng.uniform(-1, 1, len(d)).astype(dtype)) if inc else None
+            u0 = None if tu is None else tu.cpu().numpy()
+            api.ragged_batch(dev(a), dev(b), dev(c), td, dev(off), max_len=max_len, u=tu,
+                             sync=True)
+            got = (tu if inc else td).cpu().numpy()
+            if inc:
+                assert np.array_equal(td.cpu().numpy(), d)
+            check_ragged(off, a, b, c, d, got, dtype, u0)
+    finally:
+        if ctx:
+            ctx.__exit__(None, None, None)
+
+
+def test_ragged_too_long_is_reported():
+    rng = np.random.default_rng(4)
+    off, a, b, c, d = ragged_case(rng, [10, 9000, 5], np.float64)
+    td = dev(d)
+    with pytest.raises(Exception):
+        api.ragged_batch(dev(a), dev(b), dev(c), td, dev(off), max_len=8192, sync=True)
+    with pytest.raises(Exception):  # max_len beyond what the kernel supports
+        api.ragged_batch(dev(a), dev(b), dev(c), td, dev(off), max_len=100000, sync=True)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_ragged_graph_replay(dtype):
+    """Capture the planning kernels and the persistent solve once; replay it
+    several times with eager work in between; every replay equals the eager
+    result."""
+    rng = np.random.default_rng(9)
+    lens = np.round(np.exp(rng.uniform(np.log(8), np.log(4096), 500))).astype(np.int64)
+    off, a, b, c, d = ragged_case(rng, lens, dtype)
+    ta, tb, tc, toff = dev(a), dev(b), dev(c), dev(off)
+    t_in = dev(d)
+    eager = t_in.clone()
+    api.ragged_batch(ta, tb, tc, eager, toff, max_len=4096, sync=True)
+    s = torch.cuda.Stream()
+    work = torch.empty_like(t_in)
+    with torch.cuda.stream(s):
+        # workspace must exist before capture
+        assert api._lib.lib().tridHipSetStream(api.C.c_void_p(s.cuda_stream)) == 0
+        assert api._lib.lib().tridHipReserveWorkspace(
+            api._lib.lib().tridHipRaggedWorkspaceBytes(len(lens))) == 0
+        work.copy_(t_in)
+        api.ragged_batch(ta, tb, tc, work, toff, max_len=4096, stream=s)  # warm-up
+    s.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        api.ragged_batch(ta, tb, tc, work, toff, max_len=4096, stream=s)
+    for k in range(4):
+        work.copy_(t_in)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(work, eager), k
+        t_in.mul_(1.0)  # eager work between replays
+    api.use_torch_stream(torch.cuda.default_stream())
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_config_ragged_40k(dtype):
+    """40,000 systems, lengths round(exp(U(ln 8, ln 8192))) (FP32: up to 8192
+    with M = 8), diagonally dominant; one persistent launch vs the oracle."""
+    rng = np.random.default_rng(40000)
+    lens = np.round(np.exp(rng.uniform(np.log(8), np.log(8192), 40000))).astype(np.int64)
+    off, a, b, c, d = ragged_case(rng, lens, dtype)
+    td = dev(d)
+    api.ragged_batch(dev(a), dev(b), dev(c), td, dev(off), max_len=8192, sync=True)
+    check_ragged(off, a, b, c, d, td.cpu().numpy(), dtype)
+    print("ragged 40k %s: %d unknowns, config %s" % (np.dtype(dtype).name, off[-1],
+                                                      api.last_config()))

@@ -55,6 +55,7 @@ SIGNATURES = {
     "tridDmtsvStridedBatchInc": ([P, P, P, P, P, P, I, I, P, P], I),
     "tridSmtsvStridedBatchInc": ([P, P, P, P, P, P, I, I, P, P], I),
     # include/tridsolver_hip.h
+    "tridHipLastErrorString": ([], C.c_char_p),
     "tridHipSetStream": ([P], I),
     "tridHipGetStream": ([C.POINTER(C.c_void_p)], I),
     "tridHipReserveWorkspace": ([SZ], I),
@@ -63,6 +64,15 @@ SIGNATURES = {
     "tridHipLastConfig": ([IP, IP, IP, IP, IP], I),
     "tridHipForceHybridConfig": ([I, I, I, I], I),
     "tridHipStreamCopy": ([P, P, SZ], I),
+    "tridDmtsvStridedBatchMixed": ([P, P, P, P, P, I, I, P, P, I], I),
+    "tridDmtsvStridedBatchAdjoint": ([P] * 10 + [I, I, P, P], I),
+    "tridSmtsvStridedBatchAdjoint": ([P] * 10 + [I, I, P, P], I),
+    "tridDmtsvRaggedBatch": ([P] * 6 + [L, L], I),
+    "tridSmtsvRaggedBatch": ([P] * 6 + [L, L], I),
+    "tridDmtsvRaggedBatchInc": ([P] * 7 + [L, L], I),
+    "tridSmtsvRaggedBatchInc": ([P] * 7 + [L, L], I),
+    "tridHipRaggedWorkspaceBytes": ([L], SZ),
+    "tridHipForceRaggedConfig": ([I, I], I),
     # include/tridsolver_adi.h
     "tridAdiPreprocD": ([D] + [P] * 11 + [I, I, I], I),
     "tridAdiPreprocS": ([F] + [P] * 11 + [I, I, I], I),
@@ -98,7 +108,10 @@ def exported_symbols():
 
 def check(status, what=""):
     if status != 0:
-        raise TridError(status, what)
+        detail = ""
+        if status == 6 and _lib is not None:
+            detail = " (%s)" % _lib.tridHipLastErrorString().decode()
+        raise TridError(status, what + detail)
     return status
 
 

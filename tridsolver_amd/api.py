@@ -111,3 +111,82 @@ class forced_hybrid:
     def __exit__(self, *exc):
         _lib.lib().tridHipForceHybridConfig(0, 0, 0, 0)
         return False
+
+
+def mixed_strided_batch(a, b, c, d, *, solvedim, dims=None, pads=None, offset=0,
+                        refinements=2, sync=False, stream=None):
+    """FP64 solve by FP32 sweeps plus FP64 refinement (tridDmtsvStridedBatchMixed);
+    d (float64) is overwritten with the solution."""
+    if d.dtype != torch.float64:
+        raise TypeError("mixed precision solve takes float64 operands")
+    dims = list(d.shape[::-1]) if dims is None else dims
+    pads = (list(d.shape[::-1]) if d.dim() == len(dims) else list(dims)) if pads is None else pads
+    _validate([a, b, c, d], dims, pads, offset)
+    use_torch_stream(stream)
+    params = make_params((0, 0, 0), 1 if sync else 0)
+    st = _lib.lib().tridDmtsvStridedBatchMixed(
+        C.byref(params), *[_ptr(t, offset) for t in (a, b, c, d)], len(dims), solvedim,
+        int3(dims), int3(pads), int(refinements))
+    check(st, "tridDmtsvStridedBatchMixed")
+    return d
+
+
+def adjoint_strided_batch(a, b, c, x, w, *, solvedim, dims=None, pads=None, offset=0,
+                          lam=None, grad_a=None, grad_b=None, grad_c=None, sync=False,
+                          stream=None):
+    """Adjoint of the solve x = A^-1 d for the incoming gradient w = dL/dx.
+
+    Returns (lam, grad_a, grad_b, grad_c); lam = dL/dd is always computed,
+    each gradient only if its output tensor is given (or True to allocate).
+    """
+    dims = list(x.shape[::-1]) if dims is None else dims
+    pads = (list(x.shape[::-1]) if x.dim() == len(dims) else list(dims)) if pads is None else pads
+    lam = torch.empty_like(x) if lam is None else lam
+    outs = []
+    for g in (grad_a, grad_b, grad_c):
+        outs.append(torch.empty_like(x) if g is True else g)
+    arrs = [a, b, c, x, w, lam] + [g for g in outs if g is not None]
+    _validate(arrs, dims, pads, offset)
+    use_torch_stream(stream)
+    params = make_params((0, 0, 0), 1 if sync else 0)
+    name = "tridDmtsvStridedBatchAdjoint" if x.dtype == torch.float64 else \
+        "tridSmtsvStridedBatchAdjoint"
+    ptr = lambda t: None if t is None else _ptr(t, offset)
+    st = getattr(_lib.lib(), name)(C.byref(params), ptr(a), ptr(b), ptr(c), ptr(x), ptr(w),
+                                   ptr(lam), *[ptr(g) for g in outs], len(dims), solvedim,
+                                   int3(dims), int3(pads))
+    check(st, name)
+    return (lam, *outs)
+
+
+def ragged_batch(a, b, c, d, offsets, *, max_len, u=None, sync=False, stream=None):
+    """Ragged batch (tridXmtsvRaggedBatch[Inc]): systems back to back,
+    offsets (int64, device, nsys+1) delimit them."""
+    if offsets.dtype != torch.int64 or offsets.device != d.device:
+        raise TypeError("offsets must be an int64 tensor on the arrays' device")
+    arrs = [a, b, c, d] + ([u] if u is not None else [])
+    _validate(arrs, [d.numel()], [d.numel()], 0)
+    use_torch_stream(stream)
+    params = make_params((0, 0, 0), 1 if sync else 0)
+    pre = "tridD" if d.dtype == torch.float64 else "tridS"
+    name = pre + ("mtsvRaggedBatchInc" if u is not None else "mtsvRaggedBatch")
+    args = [_ptr(t) for t in arrs] + [_ptr(offsets)]
+    st = getattr(_lib.lib(), name)(C.byref(params), *args, offsets.numel() - 1, int(max_len))
+    check(st, name)
+    return u if u is not None else d
+
+
+class forced_ragged:
+    """Context manager forcing the ragged kernel's rows per lane and
+    workgroups per CU for this thread's calls."""
+
+    def __init__(self, M, per_cu=1):
+        self.cfg = (M, per_cu)
+
+    def __enter__(self):
+        check(_lib.lib().tridHipForceRaggedConfig(*self.cfg), "tridHipForceRaggedConfig")
+        return self
+
+    def __exit__(self, *exc):
+        _lib.lib().tridHipForceRaggedConfig(0, 1)
+        return False

@@ -1,0 +1,281 @@
+// hybrid_core.hpp - device building blocks of the Thomas-PCR hybrid kernels
+// (plain solve, mixed precision, adjoint, ragged). The algorithm is described
+// in kernels_solve.hip; this header holds its phases:
+//
+//   Lane           which system / chunk a thread owns inside its workgroup
+//   fetch / store  masked global access in "fetch order" (strided: the chunk's
+//                  own rows p*M+r; contiguous: coalesced rows p + P*r)
+//   to_chunk /     transposes between fetch order and chunk order through a
+//   to_fetch       skewed LDS tile (contiguous systems only)
+//   Ring           two LDS buffers used alternately for the lane exchanges;
+//                  one barrier per exchange is enough because a thread that
+//                  passed exchange k's barrier knows every thread finished
+//                  reading exchange k-1's buffer
+//   forward_spike / backward_spike / reduced_solve / substitute: phases (i)-(iv)
+#pragma once
+
+#include "trid_internal.h"
+
+namespace trid {
+namespace hyb {
+
+// Workgroup ids are handed to the XCDs round-robin (id % 8). Dealing ids so
+// that group q of G gets a contiguous range of system groups makes
+// neighbouring groups -- which read neighbouring 128-byte row segments -- run
+// on one XCD at the same time.
+__device__ __forceinline__ int64_t xcd_remap(int64_t bid, int64_t nb, int G) {
+  if (G <= 1) return bid;
+  const int64_t per = nb / G;
+  if (bid >= per * G) return bid;  // tail beyond a multiple of G: identity
+  return (bid % G) * per + bid / G;
+}
+
+// Skewed tile index of element e of one system: one pad slot per M elements,
+// so both the coalesced order (e = p + P*r) and the chunk order (e = p*M + r)
+// spread over the LDS banks.
+template <int M>
+__device__ __forceinline__ int skew(int e) {
+  return e + e / M;
+}
+
+struct Lane {
+  int t;      // thread id in the workgroup
+  int p;      // chunk index inside the system, 0..P-1
+  int w;      // system index inside the workgroup, 0..W-1
+  int P, W, T;
+  int pstep;  // thread-id distance between chunk p and p+1 of one system
+  int64_t base;  // element offset of the system's first row
+  int64_t N;     // system length
+  int64_t stride;
+  bool active;   // the lane belongs to an existing system
+};
+
+// Lane of a batched (Geom) launch: system groups of W x-neighbours (strided)
+// or W rows (contiguous), P lanes per system.
+template <bool CONTIG>
+__device__ __forceinline__ Lane lane_of(const Geom &g, int P, int W, int G) {
+  Lane L;
+  L.T = W * P;
+  L.t = threadIdx.x;
+  L.P = P;
+  L.W = W;
+  L.p = CONTIG ? L.t % P : L.t / W;
+  L.w = CONTIG ? L.t / P : L.t % W;
+  L.pstep = CONTIG ? 1 : W;
+  const int64_t nb0 = (g.n0 + W - 1) / W;
+  const int64_t bid = xcd_remap(blockIdx.x, gridDim.x, G);
+  const int64_t i0 = (bid % nb0) * W + L.w;
+  const int64_t i1 = bid / nb0;
+  L.active = i0 < g.n0;
+  L.base = i0 * g.st0 + i1 * g.st1;
+  L.N = g.N;
+  L.stride = g.stride;
+  return L;
+}
+
+template <int M, bool CONTIG>
+__device__ __forceinline__ int64_t fetch_row(const Lane &L, int r) {
+  return CONTIG ? (int64_t)L.p + (int64_t)L.P * r : (int64_t)L.p * M + r;
+}
+
+// v[r] = src[row(r) + shift] for the lane's fetch-order rows; rows >= N,
+// inactive lanes and shifted rows outside [lo, hi) get `fill`.
+template <int M, bool CONTIG, typename TI, typename TR>
+__device__ __forceinline__ void fetch(const Lane &L, const TI *__restrict__ src, TR (&v)[M],
+                                      TR fill, int shift, int64_t lo, int64_t hi) {
+#pragma unroll
+  for (int r = 0; r < M; ++r) {
+    const int64_t row = fetch_row<M, CONTIG>(L, r);
+    const int64_t sr = row + shift;
+    const bool ok = L.active && row < L.N && sr >= lo && sr < hi;
+    v[r] = ok ? (TR)src[L.base + sr * L.stride] : fill;
+  }
+}
+
+// Standard masked loads of a tridiagonal system: a[0] = 0, c[N-1] = 0 and
+// identity rows (a=0, b=1, c=0, d=0) beyond the length.
+template <int M, bool CONTIG, typename TI, typename TR>
+__device__ __forceinline__ void fetch_abcd(const Lane &L, const TI *a, const TI *b,
+                                           const TI *c, const TI *d, TR (&ra)[M],
+                                           TR (&rb)[M], TR (&rc)[M], TR (&rd)[M]) {
+  fetch<M, CONTIG>(L, a, ra, TR(0), 0, 1, L.N);
+  fetch<M, CONTIG>(L, b, rb, TR(1), 0, 0, L.N);
+  fetch<M, CONTIG>(L, c, rc, TR(0), 0, 0, L.N - 1);
+  fetch<M, CONTIG>(L, d, rd, TR(0), 0, 0, L.N);
+}
+
+template <int M, bool CONTIG, typename TR, typename TO>
+__device__ __forceinline__ void store(const Lane &L, TO *__restrict__ dst, const TR (&v)[M]) {
+  if (!L.active) return;
+#pragma unroll
+  for (int r = 0; r < M; ++r) {
+    const int64_t row = fetch_row<M, CONTIG>(L, r);
+    if (row < L.N) dst[L.base + row * L.stride] = (TO)v[r];
+  }
+}
+
+template <int M, bool CONTIG, typename TR, typename TO>
+__device__ __forceinline__ void store_add(const Lane &L, TO *__restrict__ dst,
+                                          const TR (&v)[M]) {
+  if (!L.active) return;
+#pragma unroll
+  for (int r = 0; r < M; ++r) {
+    const int64_t row = fetch_row<M, CONTIG>(L, r);
+    if (row < L.N) dst[L.base + row * L.stride] += (TO)v[r];
+  }
+}
+
+// Fetch order -> chunk order (contiguous systems). `tile` is this system's
+// P*(M+1)-element LDS tile.
+template <int M, bool CONTIG, typename TR>
+__device__ __forceinline__ void to_chunk(const Lane &L, TR (&v)[M], TR *tile) {
+  if (!CONTIG) return;
+#pragma unroll
+  for (int r = 0; r < M; ++r) tile[skew<M>(L.p + L.P * r)] = v[r];
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < M; ++r) v[r] = tile[L.p * (M + 1) + r];
+  __syncthreads();
+}
+
+// Chunk order -> fetch order (contiguous systems).
+template <int M, bool CONTIG, typename TR>
+__device__ __forceinline__ void to_fetch(const Lane &L, TR (&v)[M], TR *tile) {
+  if (!CONTIG) return;
+#pragma unroll
+  for (int r = 0; r < M; ++r) tile[L.p * (M + 1) + r] = v[r];
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < M; ++r) v[r] = tile[skew<M>(L.p + L.P * r)];
+  __syncthreads();
+}
+
+// Two alternating LDS buffers of `slot` bytes each.
+struct Ring {
+  unsigned char *base;
+  int slot;
+  int cur;
+  template <typename X>
+  __device__ __forceinline__ X *next() {
+    X *q = reinterpret_cast<X *>(base + (size_t)cur * slot);
+    cur ^= 1;
+    return q;
+  }
+};
+
+// Bytes of LDS a ring needs for up to `words` values of type X per thread.
+template <typename X>
+constexpr size_t ring_bytes(int T, int words) {
+  return 2 * sizeof(X) * (size_t)words * T;
+}
+
+// (i) forward sweep with left spike; (ra, rc, rd) become (al, ga, de).
+template <typename TR, int M>
+__device__ __forceinline__ void forward_spike(TR (&ra)[M], const TR (&rb)[M], TR (&rc)[M],
+                                              TR (&rd)[M]) {
+  TR inv = TR(1) / rb[0];
+  ra[0] = ra[0] * inv;
+  rc[0] = rc[0] * inv;
+  rd[0] = rd[0] * inv;
+#pragma unroll
+  for (int r = 1; r < M; ++r) {
+    const TR ar = ra[r];
+    inv = TR(1) / (rb[r] - ar * rc[r - 1]);
+    ra[r] = -ar * ra[r - 1] * inv;
+    rc[r] = rc[r] * inv;
+    rd[r] = (rd[r] - ar * rd[r - 1]) * inv;
+  }
+}
+
+// (ii) backward sweep over rows 0..M-2: (rd, ra, rc) <- (dp, ap, gp); row M-2
+// keeps its forward values. Returns the chunk's first row in reduced form.
+template <typename TR, int M>
+__device__ __forceinline__ void backward_spike(TR (&ra)[M], TR (&rc)[M], TR (&rd)[M],
+                                               TR &f_d, TR &f_a, TR &f_g) {
+#pragma unroll
+  for (int r = M - 3; r >= 0; --r) {
+    const TR gr = rc[r];
+    rd[r] = rd[r] - gr * rd[r + 1];
+    ra[r] = ra[r] - gr * ra[r + 1];
+    rc[r] = -gr * rc[r + 1];
+  }
+  if (M >= 2) {
+    f_d = rd[0]; f_a = ra[0]; f_g = rc[0];
+  } else {
+    f_d = TR(0); f_a = TR(0); f_g = TR(-1);
+  }
+}
+
+// (iii) reduced equation of this chunk from its last-row forward values and
+// chunk p+1's first row, then PCR over the P chunks of each system. Returns
+// y = x at this chunk's last row and ym = x at the previous chunk's last row.
+template <typename TR>
+__device__ __forceinline__ void reduced_solve(const Lane &L, Ring &ring, TR al_e, TR ga_e,
+                                              TR de_e, TR f_d, TR f_a, TR f_g, TR &y,
+                                              TR &ym) {
+  const int T = L.T, t = L.t, p = L.p, P = L.P, ps = L.pstep;
+  TR *q = ring.next<TR>();
+  q[t] = f_d; q[T + t] = f_a; q[2 * T + t] = f_g;
+  __syncthreads();
+  TR nd = 0, na = 0, ng = 0;
+  if (p + 1 < P) {
+    const int sl = t + ps;
+    nd = q[sl]; na = q[T + sl]; ng = q[2 * T + sl];
+  }
+  TR A = al_e, Cc = -ga_e * ng, D = de_e - ga_e * nd;
+  {
+    const TR inv = TR(1) / (TR(1) - ga_e * na);
+    A *= inv; Cc *= inv; D *= inv;
+  }
+  for (int h = 1; h < P; h <<= 1) {
+    TR *Q = ring.next<TR>();
+    Q[t] = A; Q[T + t] = Cc; Q[2 * T + t] = D;
+    __syncthreads();
+    TR Am = 0, Cm = 0, Dm = 0, Ap = 0, Cp = 0, Dp = 0;
+    if (p - h >= 0) {
+      const int sl = t - h * ps;
+      Am = Q[sl]; Cm = Q[T + sl]; Dm = Q[2 * T + sl];
+    }
+    if (p + h < P) {
+      const int sl = t + h * ps;
+      Ap = Q[sl]; Cp = Q[T + sl]; Dp = Q[2 * T + sl];
+    }
+    const TR inv = TR(1) / (TR(1) - A * Cm - Cc * Ap);
+    const TR nA = -inv * A * Am;
+    const TR nC = -inv * Cc * Cp;
+    const TR nD = inv * (D - A * Dm - Cc * Dp);
+    A = nA; Cc = nC; D = nD;
+  }
+  y = D;
+  TR *Q = ring.next<TR>();
+  Q[t] = y;
+  __syncthreads();
+  ym = (p > 0) ? Q[t - ps] : TR(0);
+}
+
+// (iv) x_r = dp_r - ap_r ym - gp_r y; x_{M-1} = y.
+template <typename TR, int M>
+__device__ __forceinline__ void substitute(const TR (&ra)[M], const TR (&rc)[M],
+                                           const TR (&rd)[M], TR y, TR ym, TR (&x)[M]) {
+#pragma unroll
+  for (int r = 0; r < M - 1; ++r) x[r] = rd[r] - ra[r] * ym - rc[r] * y;
+  x[M - 1] = y;
+}
+
+// Whole solve of the lane's chunk, given a, b, c, d in chunk order; x in
+// chunk order on return. a, c, d are overwritten.
+template <typename TR, int M>
+__device__ __forceinline__ void solve_chunk(const Lane &L, Ring &ring, TR (&ra)[M],
+                                            const TR (&rb)[M], TR (&rc)[M], TR (&rd)[M],
+                                            TR (&x)[M]) {
+  forward_spike<TR, M>(ra, rb, rc, rd);
+  const TR al_e = ra[M - 1], ga_e = rc[M - 1], de_e = rd[M - 1];
+  TR f_d, f_a, f_g;
+  backward_spike<TR, M>(ra, rc, rd, f_d, f_a, f_g);
+  TR y, ym;
+  reduced_solve<TR>(L, ring, al_e, ga_e, de_e, f_d, f_a, f_g, y, ym);
+  substitute<TR, M>(ra, rc, rd, y, ym, x);
+}
+
+}  // namespace hyb
+}  // namespace trid

@@ -1,0 +1,230 @@
+// kernels_ragged.hip - ragged batch: S independent systems of uneven lengths
+// stored back to back (system s = rows offsets[s] .. offsets[s+1]-1 of a, b,
+// c, d), solved in one persistent launch. No counterpart in the reference,
+// whose API needs equal lengths; per system the result is the plain solve's.
+//
+// Plan (four tiny kernels, all stream-ordered and capturable):
+//   reset     zero the class counters, the work counter and the error flag
+//   classify  class(s) = log2 of the lanes a system needs, P = next_pow2(ceil(L/M))
+//   prefix    per class: first list slot and first work item; items are
+//             ordered longest class first (largest-first scheduling)
+//   scatter   list[slot] = s
+// Solve: a grid of one workgroup per CU (times `per_cu`); each workgroup
+// repeatedly takes the next work item from an atomic counter -- W = T/P
+// systems of one class, P lanes each, the Thomas-PCR hybrid of
+// hybrid_core.hpp -- and leaves when the counter passes the last item, so every
+// wave exits. Systems longer than M*T set the error flag and are skipped.
+
+#include <algorithm>
+#include <atomic>
+
+#include "../../include/tridsolver_hip.h"
+#include "hybrid_core.hpp"
+#include "trid_internal.h"
+
+namespace trid {
+namespace {
+
+constexpr int RG_CLASSES = 11;  // P = 1, 2, 4, ..., 1024
+
+struct RaggedPlan {
+  int counts[RG_CLASSES];
+  int cursor[RG_CLASSES];
+  int64_t sys_prefix[RG_CLASSES + 1];
+  int64_t item_prefix[RG_CLASSES + 1];  // in scheduling order (longest first)
+  int work;
+  int error;
+  int total_items;
+};
+
+__global__ void rg_reset(RaggedPlan *plan) {
+  const int t = threadIdx.x;
+  if (t < RG_CLASSES) {
+    plan->counts[t] = 0;
+    plan->cursor[t] = 0;
+  }
+  if (t == 0) {
+    plan->work = 0;
+    plan->error = 0;
+    plan->total_items = 0;
+  }
+}
+
+__device__ __forceinline__ int rg_class(int64_t L, int M, int Tmax) {
+  if (L <= 0) return -1;
+  const int64_t need = (L + M - 1) / M;
+  if (need > Tmax) return -2;
+  int c = 0;
+  while ((int64_t(1) << c) < need) ++c;
+  return c;
+}
+
+__global__ void rg_classify(const int64_t *__restrict__ off, int64_t S, int M, int Tmax,
+                            RaggedPlan *plan) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  const int c = rg_class(off[s + 1] - off[s], M, Tmax);
+  if (c >= 0) atomicAdd(&plan->counts[c], 1);
+  else if (c == -2) atomicOr(&plan->error, 1);
+}
+
+__global__ void rg_prefix(RaggedPlan *plan, int Tmax) {
+  if (threadIdx.x != 0) return;
+  int64_t sp = 0;
+  for (int c = 0; c < RG_CLASSES; ++c) {
+    plan->sys_prefix[c] = sp;
+    sp += plan->counts[c];
+  }
+  plan->sys_prefix[RG_CLASSES] = sp;
+  // item_prefix[k] for scheduling slot k = class RG_CLASSES-1-k
+  int64_t ip = 0;
+  for (int k = 0; k < RG_CLASSES; ++k) {
+    const int c = RG_CLASSES - 1 - k;
+    plan->item_prefix[k] = ip;
+    const int W = Tmax >> c;
+    ip += W > 0 ? (plan->counts[c] + W - 1) / W : 0;
+  }
+  plan->item_prefix[RG_CLASSES] = ip;
+  plan->total_items = (int)ip;
+}
+
+__global__ void rg_scatter(const int64_t *__restrict__ off, int64_t S, int M, int Tmax,
+                           RaggedPlan *plan, int *__restrict__ list) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  const int c = rg_class(off[s + 1] - off[s], M, Tmax);
+  if (c < 0) return;
+  const int pos = atomicAdd(&plan->cursor[c], 1);
+  list[plan->sys_prefix[c] + pos] = (int)s;
+}
+
+// Threads per workgroup: the register file holds 1024 threads of the M = 4/8
+// variants and 512 of FP64 M = 16 (FP32 M = 16 is not built: it spills at 1024).
+template <typename REAL, int M>
+struct RgBounds {
+  static constexpr int threads = sizeof(REAL) * M > 64 ? 512 : 1024;
+};
+
+template <typename REAL, int M, bool INC>
+__global__ __launch_bounds__((RgBounds<REAL, M>::threads)) void rg_solve(
+    const REAL *__restrict__ a, const REAL *__restrict__ b, const REAL *__restrict__ c,
+    REAL *__restrict__ d, REAL *__restrict__ u, const int64_t *__restrict__ off,
+    const int *__restrict__ list, RaggedPlan *plan) {
+  constexpr int Tmax = RgBounds<REAL, M>::threads;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ int s_item;
+  const int t = threadIdx.x;
+  const int total = plan->total_items;
+  for (;;) {
+    if (t == 0) s_item = atomicAdd(&plan->work, 1);
+    __syncthreads();
+    const int item = s_item;
+    __syncthreads();  // everyone has read s_item before it changes
+    if (item >= total) break;
+    int k = 0;
+    while (k + 1 < RG_CLASSES && plan->item_prefix[k + 1] <= item) ++k;
+    const int cls = RG_CLASSES - 1 - k;
+    hyb::Lane L;
+    L.P = 1 << cls;
+    L.W = Tmax >> cls;
+    L.T = Tmax;
+    L.t = t;
+    L.p = t % L.P;
+    L.w = t / L.P;
+    L.pstep = 1;
+    L.stride = 1;
+    const int64_t first = plan->sys_prefix[cls] + (int64_t)(item - plan->item_prefix[k]) * L.W;
+    const int64_t slot = first + L.w;
+    L.active = slot < plan->sys_prefix[cls + 1];
+    int64_t sys = L.active ? list[slot] : 0;
+    L.base = L.active ? off[sys] : 0;
+    L.N = L.active ? off[sys + 1] - off[sys] : 0;
+
+    REAL *tile = reinterpret_cast<REAL *>(smem_raw) + (size_t)L.w * L.P * (M + 1);
+    hyb::Ring ring{smem_raw + sizeof(REAL) * (size_t)Tmax * (M + 1),
+                   (int)(3 * sizeof(REAL) * Tmax), 0};
+    REAL ra[M], rb[M], rc[M], rd[M];
+    hyb::fetch_abcd<M, true>(L, a, b, c, d, ra, rb, rc, rd);
+    hyb::to_chunk<M, true>(L, ra, tile);
+    hyb::to_chunk<M, true>(L, rb, tile);
+    hyb::to_chunk<M, true>(L, rc, tile);
+    hyb::to_chunk<M, true>(L, rd, tile);
+    REAL x[M];
+    hyb::solve_chunk<REAL, M>(L, ring, ra, rb, rc, rd, x);
+    hyb::to_fetch<M, true>(L, x, tile);
+    if (INC) hyb::store_add<M, true>(L, u, x);
+    else hyb::store<M, true>(L, d, x);
+  }
+}
+
+template <typename REAL, int M, bool INC>
+hipError_t launch_m(const REAL *a, const REAL *b, const REAL *c, REAL *d, REAL *u,
+                    const int64_t *off, int64_t S, int per_cu, void *ws, hipStream_t st,
+                    int dev) {
+  constexpr int Tmax = RgBounds<REAL, M>::threads;
+  RaggedPlan *plan = static_cast<RaggedPlan *>(ws);
+  int *list = reinterpret_cast<int *>(static_cast<unsigned char *>(ws) + 512);
+  hipLaunchKernelGGL(rg_reset, dim3(1), dim3(64), 0, st, plan);
+  const unsigned nb = (unsigned)((S + 255) / 256);
+  if (nb > 0) {
+    hipLaunchKernelGGL(rg_classify, dim3(nb), dim3(256), 0, st, off, S, M, Tmax, plan);
+  }
+  hipLaunchKernelGGL(rg_prefix, dim3(1), dim3(64), 0, st, plan, Tmax);
+  if (nb > 0) {
+    hipLaunchKernelGGL(rg_scatter, dim3(nb), dim3(256), 0, st, off, S, M, Tmax, plan, list);
+  }
+  const size_t lds = sizeof(REAL) * (size_t)Tmax * (M + 1) + hyb::ring_bytes<REAL>(Tmax, 3);
+  auto kern = rg_solve<REAL, M, INC>;
+  static std::atomic<size_t> lds_granted{0};
+  {
+    hipError_t e = ensure_lds(reinterpret_cast<const void *>(kern), lds, lds_granted);
+    if (e != hipSuccess) return e;
+  }
+  int cus = 256;
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  hipLaunchKernelGGL(kern, dim3((unsigned)(cus * std::max(1, per_cu))), dim3(Tmax), lds, st,
+                     a, b, c, d, u, off, list, plan);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+size_t ragged_workspace_bytes(int64_t S) { return 512 + sizeof(int) * (size_t)std::max<int64_t>(S, 1); }
+
+int ragged_max_len(int fp64, int M) {
+  if (M != 4 && M != 8 && !(M == 16 && fp64)) return 0;
+  const int T = (fp64 ? 8 : 4) * M > 64 ? 512 : 1024;
+  return M * T;
+}
+
+template <typename REAL, bool INC>
+hipError_t launch_ragged(int M, const REAL *a, const REAL *b, const REAL *c, REAL *d, REAL *u,
+                         const int64_t *off, int64_t S, int per_cu, void *ws, hipStream_t st,
+                         int dev) {
+  static_assert(sizeof(RaggedPlan) <= 512, "plan header");
+  switch (M) {
+    case 4: return launch_m<REAL, 4, INC>(a, b, c, d, u, off, S, per_cu, ws, st, dev);
+    case 8: return launch_m<REAL, 8, INC>(a, b, c, d, u, off, S, per_cu, ws, st, dev);
+    case 16:
+      if (sizeof(REAL) == 8) return launch_m<REAL, 16, INC>(a, b, c, d, u, off, S, per_cu, ws, st, dev);
+      return hipErrorInvalidValue;
+    default: return hipErrorInvalidValue;
+  }
+}
+
+// Reads the plan's error flag (after the stream has completed).
+hipError_t ragged_error_flag(void *ws, int *flag) {
+  RaggedPlan *plan = static_cast<RaggedPlan *>(ws);
+  return hipMemcpy(flag, &plan->error, sizeof(int), hipMemcpyDeviceToHost);
+}
+
+#define TRID_RG_INST(REAL, INC)                                                              \
+  template hipError_t launch_ragged<REAL, INC>(int, const REAL *, const REAL *, const REAL *, \
+                                               REAL *, REAL *, const int64_t *, int64_t, int, \
+                                               void *, hipStream_t, int);
+TRID_RG_INST(float, false)
+TRID_RG_INST(float, true)
+TRID_RG_INST(double, false)
+TRID_RG_INST(double, true)
+
+}  // namespace trid

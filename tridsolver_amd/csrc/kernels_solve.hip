@@ -49,6 +49,7 @@
 #include <algorithm>
 #include <atomic>
 
+#include "hybrid_core.hpp"
 #include "trid_internal.h"
 
 namespace trid {
@@ -148,196 +149,28 @@ constexpr int hybrid_max_threads() {
   return HybridBounds<REAL, M>::threads;
 }
 
-// Maps the hardware workgroup id to the system group it solves. Workgroups
-// are handed to the XCDs round-robin (id % 8); dealing ids so that group q of
-// G gets a contiguous range makes neighbouring system groups -- which read
-// neighbouring 128-byte row segments -- run on one XCD at the same time.
-__device__ __forceinline__ int64_t xcd_remap(int64_t bid, int64_t nb, int G) {
-  if (G <= 1) return bid;
-  const int64_t per = nb / G;
-  if (bid >= per * G) return bid;  // tail beyond a multiple of G: identity
-  return (bid % G) * per + bid / G;
-}
-
-// Skewed LDS tile index of element e of one system (one pad slot per M
-// elements) so that both the coalesced order (e = p + P*r) and the chunk order
-// (e = p*M + r) are spread over the banks.
-template <int M>
-__device__ __forceinline__ int skew(int e) {
-  return e + e / M;
-}
-
 template <typename REAL, int M, bool INC, bool CONTIG>
 __global__ __launch_bounds__((HybridBounds<REAL, M>::threads)) void hybrid_kernel(
     Geom g, int P, int W, int G, const REAL *__restrict__ a,
     const REAL *__restrict__ b, const REAL *__restrict__ c, REAL *__restrict__ d,
     REAL *__restrict__ u) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  REAL *smem = reinterpret_cast<REAL *>(smem_raw);
-  const int T = W * P;
-  const int t = threadIdx.x;
-  const int p = CONTIG ? t % P : t / W;
-  const int w = CONTIG ? t / P : t % W;
-  const int pstep = CONTIG ? 1 : W;  // slot distance between chunk p and p+1
-
-  const int64_t nb0 = (g.n0 + W - 1) / W;
-  const int64_t bid = xcd_remap(blockIdx.x, gridDim.x, G);
-  const int64_t i0 = (bid % nb0) * W + w;
-  const int64_t i1 = bid / nb0;
-  const bool active = i0 < g.n0;
-  const int64_t base = i0 * g.st0 + i1 * g.st1;
-  const int64_t N = g.N;
-  const int SK = P * (M + 1);  // skewed tile length of one system (contig)
-  REAL *tile = smem + (int64_t)w * SK;
-  REAL *pcr = smem + (CONTIG ? (int64_t)W * SK : 0);  // 2 buffers x 3 x T
+  const hyb::Lane L = hyb::lane_of<CONTIG>(g, P, W, G);
+  const size_t tile_bytes = CONTIG ? sizeof(REAL) * (size_t)W * P * (M + 1) : 0;
+  REAL *tile = reinterpret_cast<REAL *>(smem_raw) + (size_t)L.w * P * (M + 1);
+  hyb::Ring ring{smem_raw + tile_bytes, (int)(3 * sizeof(REAL) * L.T), 0};
 
   REAL ra[M], rb[M], rc[M], rd[M];
-  if (CONTIG) {
-    REAL va[M], vb[M], vc[M], vd[M];
-#pragma unroll
-    for (int r = 0; r < M; ++r) {
-      const int64_t e = p + (int64_t)P * r;
-      const bool ok = active && e < N;
-      const int64_t o = base + e;
-      va[r] = (ok && e > 0) ? a[o] : REAL(0);
-      vb[r] = ok ? b[o] : REAL(1);
-      vc[r] = (ok && e < N - 1) ? c[o] : REAL(0);
-      vd[r] = ok ? d[o] : REAL(0);
-    }
-    auto transpose_in = [&](const REAL(&v)[M], REAL(&dst)[M]) {
-#pragma unroll
-      for (int r = 0; r < M; ++r) tile[skew<M>(p + P * r)] = v[r];
-      __syncthreads();
-#pragma unroll
-      for (int r = 0; r < M; ++r) dst[r] = tile[p * (M + 1) + r];
-      __syncthreads();
-    };
-    transpose_in(va, ra);
-    transpose_in(vb, rb);
-    transpose_in(vc, rc);
-    transpose_in(vd, rd);
-  } else {
-#pragma unroll
-    for (int r = 0; r < M; ++r) {
-      const int64_t row = (int64_t)p * M + r;
-      const bool ok = active && row < N;
-      const int64_t o = base + row * g.stride;
-      ra[r] = (ok && row > 0) ? a[o] : REAL(0);
-      rb[r] = ok ? b[o] : REAL(1);
-      rc[r] = (ok && row < N - 1) ? c[o] : REAL(0);
-      rd[r] = ok ? d[o] : REAL(0);
-    }
-  }
-
-  // (i) forward sweep with left spike; al/ga/de overwrite ra/rc/rd.
-  {
-    REAL inv = REAL(1) / rb[0];
-    ra[0] = ra[0] * inv;
-    rc[0] = rc[0] * inv;
-    rd[0] = rd[0] * inv;
-#pragma unroll
-    for (int r = 1; r < M; ++r) {
-      const REAL ar = ra[r];
-      inv = REAL(1) / (rb[r] - ar * rc[r - 1]);
-      ra[r] = -ar * ra[r - 1] * inv;
-      rc[r] = rc[r] * inv;
-      rd[r] = (rd[r] - ar * rd[r - 1]) * inv;
-    }
-  }
-  // Forward values of the chunk's last row, used by the reduced equation.
-  const REAL al_e = ra[M - 1], ga_e = rc[M - 1], de_e = rd[M - 1];
-
-  // (ii) backward sweep over rows 0..M-2: (rd, ra, rc) <- (dp, ap, gp).
-  // Row M-2 keeps its forward values (dp, ap, gp of row e are (0, 0, -1)).
-#pragma unroll
-  for (int r = M - 3; r >= 0; --r) {
-    const REAL gr = rc[r];
-    rd[r] = rd[r] - gr * rd[r + 1];
-    ra[r] = ra[r] - gr * ra[r + 1];
-    rc[r] = -gr * rc[r + 1];
-  }
-  REAL f_d, f_a, f_g;  // first row of this chunk in reduced form
-  if (M >= 2) {
-    f_d = rd[0]; f_a = ra[0]; f_g = rc[0];
-  } else {
-    f_d = REAL(0); f_a = REAL(0); f_g = REAL(-1);
-  }
-
-  // (iii) reduced equation of chunk p; needs chunk p+1's first row.
-  REAL *q0 = pcr, *q1 = pcr + 3 * T;
-  q0[t] = f_d; q0[T + t] = f_a; q0[2 * T + t] = f_g;
-  __syncthreads();
-  REAL nd = 0, na = 0, ng = 0;
-  if (p + 1 < P) {
-    const int sl = t + pstep;
-    nd = q0[sl]; na = q0[T + sl]; ng = q0[2 * T + sl];
-  }
-  REAL A = al_e, Cc = -ga_e * ng, D = de_e - ga_e * nd;
-  {
-    const REAL inv = REAL(1) / (REAL(1) - ga_e * na);
-    A *= inv; Cc *= inv; D *= inv;
-  }
-  // PCR: after the step of distance h, rows couple at distance 2h.
-  REAL *Q = q1;
-  for (int h = 1; h < P; h <<= 1) {
-    Q[t] = A; Q[T + t] = Cc; Q[2 * T + t] = D;
-    __syncthreads();
-    REAL Am = 0, Cm = 0, Dm = 0, Ap = 0, Cp = 0, Dp = 0;
-    if (p - h >= 0) {
-      const int sl = t - h * pstep;
-      Am = Q[sl]; Cm = Q[T + sl]; Dm = Q[2 * T + sl];
-    }
-    if (p + h < P) {
-      const int sl = t + h * pstep;
-      Ap = Q[sl]; Cp = Q[T + sl]; Dp = Q[2 * T + sl];
-    }
-    const REAL inv = REAL(1) / (REAL(1) - A * Cm - Cc * Ap);
-    const REAL nA = -inv * A * Am;
-    const REAL nC = -inv * Cc * Cp;
-    const REAL nD = inv * (D - A * Dm - Cc * Dp);
-    A = nA; Cc = nC; D = nD;
-    Q = (Q == q1) ? q0 : q1;
-  }
-  const REAL y = D;  // x at the last row of chunk p
-  Q[t] = y;
-  __syncthreads();
-  const REAL ym = (p > 0) ? Q[t - pstep] : REAL(0);
-
-  // (iv) interior rows
+  hyb::fetch_abcd<M, CONTIG>(L, a, b, c, d, ra, rb, rc, rd);
+  hyb::to_chunk<M, CONTIG>(L, ra, tile);
+  hyb::to_chunk<M, CONTIG>(L, rb, tile);
+  hyb::to_chunk<M, CONTIG>(L, rc, tile);
+  hyb::to_chunk<M, CONTIG>(L, rd, tile);
   REAL x[M];
-#pragma unroll
-  for (int r = 0; r < M - 1; ++r) x[r] = rd[r] - ra[r] * ym - rc[r] * y;
-  x[M - 1] = y;
-
-  if (CONTIG) {
-    // Q was last written before the barrier above; the tile region is
-    // disjoint from the PCR buffers, and every lane finished reading the tile
-    // before the reduced system started.
-#pragma unroll
-    for (int r = 0; r < M; ++r) tile[p * (M + 1) + r] = x[r];
-    __syncthreads();
-    if (active) {
-#pragma unroll
-      for (int r = 0; r < M; ++r) {
-        const int64_t e = p + (int64_t)P * r;
-        if (e < N) {
-          const REAL v = tile[skew<M>((int)e)];
-          if (INC) u[base + e] += v;
-          else d[base + e] = v;
-        }
-      }
-    }
-  } else if (active) {
-#pragma unroll
-    for (int r = 0; r < M; ++r) {
-      const int64_t row = (int64_t)p * M + r;
-      if (row < N) {
-        const int64_t o = base + row * g.stride;
-        if (INC) u[o] += x[r];
-        else d[o] = x[r];
-      }
-    }
-  }
+  hyb::solve_chunk<REAL, M>(L, ring, ra, rb, rc, rd, x);
+  hyb::to_fetch<M, CONTIG>(L, x, tile);
+  if (INC) hyb::store_add<M, CONTIG>(L, u, x);
+  else hyb::store<M, CONTIG>(L, d, x);
 }
 
 template <typename REAL, int M, bool INC, bool CONTIG>
@@ -348,15 +181,10 @@ hipError_t launch_hybrid_m(const Geom &g, const HybridCfg &cfg, const REAL *a,
   size_t lds = sizeof(REAL) * (size_t)6 * T;
   if (CONTIG) lds += sizeof(REAL) * (size_t)cfg.W * cfg.P * (M + 1);
   auto kern = hybrid_kernel<REAL, M, INC, CONTIG>;
-  // Raise the dynamic LDS limit once per kernel (not capturable, so it is not
-  // repeated inside a hipGraph capture of a later call).
-  static std::atomic<int> lds_raised{0};
-  if (lds > 65536 && !lds_raised.load()) {
-    hipError_t e = hipFuncSetAttribute(
-        reinterpret_cast<const void *>(kern),
-        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  static std::atomic<size_t> lds_granted{0};
+  {
+    hipError_t e = ensure_lds(reinterpret_cast<const void *>(kern), lds, lds_granted);
     if (e != hipSuccess) return e;
-    lds_raised.store(1);
   }
   const int64_t nb0 = (g.n0 + cfg.W - 1) / cfg.W;
   const int64_t blocks = nb0 * g.n1;

@@ -16,6 +16,7 @@
 #include <map>
 #include <mutex>
 #include <utility>
+#include <vector>
 
 #include "../../include/tridsolver.h"
 #include "../../include/tridsolver_hip.h"
@@ -29,6 +30,7 @@ thread_local hipStream_t tl_stream = nullptr;
 thread_local int tl_last_algo = 0, tl_last_m = 0, tl_last_p = 0, tl_last_w = 0,
                  tl_last_g = 0;
 thread_local HybridCfg tl_forced;
+thread_local hipError_t tl_hip_error = hipSuccess;
 
 std::mutex g_mu;
 struct Workspace {
@@ -36,6 +38,9 @@ struct Workspace {
   size_t bytes = 0;
 };
 std::map<std::pair<int, hipStream_t>, Workspace> g_ws;
+// Buffers replaced by a larger one. A hipGraph captured earlier may still
+// point at them, so they are only freed by tridHipReleaseWorkspaces().
+std::map<int, std::vector<void *>> g_retired;
 std::map<int, tridStatus_t> g_arch;
 
 }  // namespace
@@ -74,9 +79,7 @@ tridStatus_t get_workspace(int dev, hipStream_t st, size_t bytes, void **out) {
   if (hipStreamIsCapturing(st, &cap) != hipSuccess) return TRID_STATUS_EXECUTION_FAILED;
   if (cap != hipStreamCaptureStatusNone) return TRID_STATUS_INTERNAL_ERROR;
   if (ws.ptr) {
-    // The old buffer may still be read by queued work on this stream.
-    if (hipStreamSynchronize(st) != hipSuccess) return TRID_STATUS_EXECUTION_FAILED;
-    (void)hipFree(ws.ptr);
+    g_retired[dev].push_back(ws.ptr);
     ws.ptr = nullptr;
     ws.bytes = 0;
   }
@@ -119,10 +122,21 @@ tridStatus_t make_geom(int ndim, int solvedim, const int *dims, const int *pads,
   return TRID_STATUS_SUCCESS;
 }
 
+void record_hip_error(hipError_t e) { tl_hip_error = e; }
+
 tridStatus_t finish(const TridParams *ctx, hipStream_t st, hipError_t launched) {
-  if (launched != hipSuccess) return TRID_STATUS_EXECUTION_FAILED;
+  if (launched != hipSuccess) {
+    record_hip_error(launched);
+    return TRID_STATUS_EXECUTION_FAILED;
+  }
   const bool sync = ctx == nullptr || ctx->sync != 0;
-  if (sync && hipStreamSynchronize(st) != hipSuccess) return TRID_STATUS_EXECUTION_FAILED;
+  if (sync) {
+    const hipError_t e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+      record_hip_error(e);
+      return TRID_STATUS_EXECUTION_FAILED;
+    }
+  }
   return TRID_STATUS_SUCCESS;
 }
 
@@ -216,6 +230,10 @@ EXTERN_C tridStatus_t tridSmtsvStridedBatchInc(const TridParams *ctx, const floa
 // ---------------------------------------------------------------------------
 // Extensions
 // ---------------------------------------------------------------------------
+EXTERN_C const char *tridHipLastErrorString(void) {
+  return tl_hip_error == hipSuccess ? "no error" : hipGetErrorString(tl_hip_error);
+}
+
 EXTERN_C tridStatus_t tridHipSetStream(void *stream) {
   tl_stream = static_cast<hipStream_t>(stream);
   return TRID_STATUS_SUCCESS;
@@ -240,6 +258,8 @@ EXTERN_C tridStatus_t tridHipReleaseWorkspaces(void) {
   if (hipGetDevice(&dev) != hipSuccess) return TRID_STATUS_NOT_INITIALIZED;
   if (hipDeviceSynchronize() != hipSuccess) return TRID_STATUS_EXECUTION_FAILED;
   std::lock_guard<std::mutex> lk(g_mu);
+  for (void *p : g_retired[dev]) (void)hipFree(p);
+  g_retired[dev].clear();
   for (auto it = g_ws.begin(); it != g_ws.end();) {
     if (it->first.first == dev) {
       (void)hipFree(it->second.ptr);

@@ -1,0 +1,163 @@
+// trid_api_ext.hip - C ABI entry points of the extensions built on the
+// batched solve: FP32 sweeps with FP64 refinement, the adjoint, and ragged
+// batches. None has a counterpart in the reference (see include/tridsolver_hip.h).
+
+#include <algorithm>
+#include <mutex>
+
+#include "../../include/tridsolver_hip.h"
+#include "trid_internal.h"
+#include "trid_tune.h"
+
+namespace trid {
+namespace {
+
+thread_local int tl_ragged_m = 0, tl_ragged_per_cu = 1;
+
+template <typename REAL, bool INC>
+tridStatus_t ragged(const TridParams *ctx, const REAL *a, const REAL *b, const REAL *c, REAL *d,
+                    REAL *u, const int64_t *offsets, int64_t nsys, int64_t max_len) {
+  if (!a || !b || !c || !d || !offsets || (INC && !u) || nsys < 0 || max_len < 0)
+    return TRID_STATUS_INVALID_VALUE;
+  int dev = 0;
+  tridStatus_t s = check_device(&dev);
+  if (s != TRID_STATUS_SUCCESS) return s;
+  hipStream_t st = current_stream();
+  if (nsys == 0) return finish(ctx, st, hipSuccess);
+  int M = tl_ragged_m, per_cu = tl_ragged_per_cu;
+  if (M == 0) ragged_tuned_config<REAL>(dev, max_len, &M, &per_cu);
+  const int supported = ragged_max_len(sizeof(REAL) == 8, M);
+  if (supported == 0 || max_len > supported) return TRID_STATUS_INVALID_VALUE;
+  void *ws = nullptr;
+  if ((s = get_workspace(dev, st, ragged_workspace_bytes(nsys), &ws)) != TRID_STATUS_SUCCESS)
+    return s;
+  HybridCfg rec;
+  rec.M = M; rec.P = 0; rec.W = 0; rec.G = per_cu;
+  record_config(5, &rec);
+  s = finish(ctx, st, launch_ragged<REAL, INC>(M, a, b, c, d, u, offsets, nsys, per_cu, ws, st,
+                                              dev));
+  if (s == TRID_STATUS_SUCCESS && (ctx == nullptr || ctx->sync != 0)) {
+    int flag = 0;
+    if (ragged_error_flag(ws, &flag) != hipSuccess) return TRID_STATUS_EXECUTION_FAILED;
+    if (flag) return TRID_STATUS_INVALID_VALUE;  // a system longer than supported
+  }
+  return s;
+}
+
+template <typename REAL>
+tridStatus_t adjoint(const TridParams *ctx, const REAL *a, const REAL *b, const REAL *c,
+                     const REAL *x, const REAL *w, REAL *lam, REAL *ga, REAL *gb, REAL *gc,
+                     int ndim, int solvedim, const int *dims, const int *pads) {
+  Geom g;
+  bool empty = false;
+  tridStatus_t s = make_geom(ndim, solvedim, dims, pads, &g, &empty);
+  if (s != TRID_STATUS_SUCCESS) return s;
+  if (!a || !b || !c || !x || !w || !lam) return TRID_STATUS_INVALID_VALUE;
+  int dev = 0;
+  if ((s = check_device(&dev)) != TRID_STATUS_SUCCESS) return s;
+  hipStream_t st = current_stream();
+  if (empty) return finish(ctx, st, hipSuccess);
+  HybridCfg cfg;
+  const HybridCfg &forced = forced_config();
+  if (forced.M != 0) {
+    cfg = forced;
+    if (!adjoint_cfg_valid<REAL>(g, cfg)) return TRID_STATUS_INVALID_VALUE;
+  } else if (!adjoint_default_cfg<REAL>(g, &cfg)) {
+    return TRID_STATUS_INVALID_VALUE;  // longer than one workgroup can hold
+  }
+  record_config(4, &cfg);
+  return finish(ctx, st, launch_adjoint<REAL>(g, cfg, a, b, c, x, w, lam, ga, gb, gc, st));
+}
+
+}  // namespace
+}  // namespace trid
+
+using namespace trid;
+
+EXTERN_C tridStatus_t tridDmtsvStridedBatchMixed(const TridParams *ctx, const double *a,
+                                                 const double *b, const double *c, double *d,
+                                                 int ndim, int solvedim, const int *dims,
+                                                 const int *pads, int refinements) {
+  Geom g;
+  bool empty = false;
+  tridStatus_t s = make_geom(ndim, solvedim, dims, pads, &g, &empty);
+  if (s != TRID_STATUS_SUCCESS) return s;
+  if (!a || !b || !c || !d || refinements < 0 || refinements > 8) return TRID_STATUS_INVALID_VALUE;
+  int dev = 0;
+  if ((s = check_device(&dev)) != TRID_STATUS_SUCCESS) return s;
+  hipStream_t st = current_stream();
+  if (empty) return finish(ctx, st, hipSuccess);
+  HybridCfg cfg;
+  const HybridCfg &forced = forced_config();
+  if (forced.M != 0) {
+    cfg = forced;
+    if (!mixed_cfg_valid(g, cfg)) return TRID_STATUS_INVALID_VALUE;
+  } else if (!mixed_tuned_config(dev, g, &cfg)) {
+    return TRID_STATUS_INVALID_VALUE;
+  }
+  record_config(2, &cfg);
+  return finish(ctx, st, launch_mixed(g, cfg, refinements, a, b, c, d, st));
+}
+
+EXTERN_C tridStatus_t tridDmtsvStridedBatchAdjoint(const TridParams *ctx, const double *a,
+                                                   const double *b, const double *c,
+                                                   const double *x, const double *w,
+                                                   double *lam, double *grad_a,
+                                                   double *grad_b, double *grad_c, int ndim,
+                                                   int solvedim, const int *dims,
+                                                   const int *pads) {
+  return adjoint<double>(ctx, a, b, c, x, w, lam, grad_a, grad_b, grad_c, ndim, solvedim,
+                         dims, pads);
+}
+
+EXTERN_C tridStatus_t tridSmtsvStridedBatchAdjoint(const TridParams *ctx, const float *a,
+                                                   const float *b, const float *c,
+                                                   const float *x, const float *w, float *lam,
+                                                   float *grad_a, float *grad_b, float *grad_c,
+                                                   int ndim, int solvedim, const int *dims,
+                                                   const int *pads) {
+  return adjoint<float>(ctx, a, b, c, x, w, lam, grad_a, grad_b, grad_c, ndim, solvedim, dims,
+                        pads);
+}
+
+EXTERN_C tridStatus_t tridDmtsvRaggedBatch(const TridParams *ctx, const double *a,
+                                           const double *b, const double *c, double *d,
+                                           const int64_t *offsets, int64_t nsys,
+                                           int64_t max_len) {
+  return ragged<double, false>(ctx, a, b, c, d, nullptr, offsets, nsys, max_len);
+}
+
+EXTERN_C tridStatus_t tridSmtsvRaggedBatch(const TridParams *ctx, const float *a,
+                                           const float *b, const float *c, float *d,
+                                           const int64_t *offsets, int64_t nsys,
+                                           int64_t max_len) {
+  return ragged<float, false>(ctx, a, b, c, d, nullptr, offsets, nsys, max_len);
+}
+
+EXTERN_C tridStatus_t tridDmtsvRaggedBatchInc(const TridParams *ctx, const double *a,
+                                              const double *b, const double *c, double *d,
+                                              double *u, const int64_t *offsets, int64_t nsys,
+                                              int64_t max_len) {
+  return ragged<double, true>(ctx, a, b, c, d, u, offsets, nsys, max_len);
+}
+
+EXTERN_C tridStatus_t tridSmtsvRaggedBatchInc(const TridParams *ctx, const float *a,
+                                              const float *b, const float *c, float *d,
+                                              float *u, const int64_t *offsets, int64_t nsys,
+                                              int64_t max_len) {
+  return ragged<float, true>(ctx, a, b, c, d, u, offsets, nsys, max_len);
+}
+
+EXTERN_C tridStatus_t tridHipForceRaggedConfig(int M, int per_cu) {
+  if (M == 0) {
+    tl_ragged_m = 0;
+    tl_ragged_per_cu = 1;
+    return TRID_STATUS_SUCCESS;
+  }
+  if ((M != 4 && M != 8 && M != 16) || per_cu < 1 || per_cu > 8) return TRID_STATUS_INVALID_VALUE;
+  tl_ragged_m = M;
+  tl_ragged_per_cu = per_cu;
+  return TRID_STATUS_SUCCESS;
+}
+
+EXTERN_C size_t tridHipRaggedWorkspaceBytes(int64_t nsys) { return ragged_workspace_bytes(nsys); }

@@ -3,10 +3,12 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <atomic>
 #include <cstddef>
 #include <cstdint>
 
 #include "../../include/trid_common.h"
+#include "../../include/trid_params.h"
 
 namespace trid {
 
@@ -58,6 +60,50 @@ hipError_t launch_hybrid(const Geom &g, const HybridCfg &cfg, const REAL *a,
 hipStream_t current_stream();
 tridStatus_t check_device(int *dev);
 tridStatus_t get_workspace(int dev, hipStream_t st, size_t bytes, void **out);
+
+// ---- mixed precision (kernels_mixed.hip), FP64 operands ------------------
+bool mixed_cfg_valid(const Geom &g, const HybridCfg &c);
+bool mixed_default_cfg(const Geom &g, HybridCfg *cfg);
+hipError_t launch_mixed(const Geom &g, const HybridCfg &cfg, int refine, const double *a,
+                        const double *b, const double *c, double *d, hipStream_t st);
+
+// ---- adjoint (kernels_adjoint.hip) ----------------------------------------
+template <typename REAL>
+bool adjoint_cfg_valid(const Geom &g, const HybridCfg &c);
+template <typename REAL>
+bool adjoint_default_cfg(const Geom &g, HybridCfg *cfg);
+template <typename REAL>
+hipError_t launch_adjoint(const Geom &g, const HybridCfg &cfg, const REAL *a, const REAL *b,
+                          const REAL *c, const REAL *x, const REAL *w, REAL *lam, REAL *ga,
+                          REAL *gb, REAL *gc, hipStream_t st);
+
+// ---- ragged batch (kernels_ragged.hip) ------------------------------------
+size_t ragged_workspace_bytes(int64_t S);
+int ragged_max_len(int fp64, int M);
+template <typename REAL, bool INC>
+hipError_t launch_ragged(int M, const REAL *a, const REAL *b, const REAL *c, REAL *d, REAL *u,
+                         const int64_t *off, int64_t S, int per_cu, void *ws, hipStream_t st,
+                         int dev);
+hipError_t ragged_error_flag(void *ws, int *flag);
+
+// Raises a kernel's dynamic-LDS limit to `lds` bytes when it exceeds the
+// default 64 KiB and what was granted before (not capturable: the first call
+// with a given size must happen outside hipGraph capture).
+inline hipError_t ensure_lds(const void *kern, size_t lds, std::atomic<size_t> &granted) {
+  if (lds <= 65536 || lds <= granted.load()) return hipSuccess;
+  hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e == hipSuccess) granted.store(lds);
+  return e;
+}
+
+// ---- shared entry-point helpers (trid_api.hip) ----------------------------
+// Records the HIP error behind a failed call (tridHipLastErrorString()).
+void record_hip_error(hipError_t e);
+tridStatus_t make_geom(int ndim, int solvedim, const int *dims, const int *pads, Geom *g,
+                       bool *empty);
+tridStatus_t finish(const TridParams *ctx, hipStream_t st, hipError_t launched);
+void record_config(int algo, const HybridCfg *cfg);
+const HybridCfg &forced_config();
 
 // Picks a valid hybrid configuration for g, or returns false if the systems
 // are too long for one workgroup.

@@ -10,4 +10,12 @@ namespace trid {
 template <typename REAL, bool INC>
 bool tuned_config(int dev, const Geom &g, HybridCfg *cfg);
 
+// Shape of the fused FP32/FP64-refinement kernel for g.
+bool mixed_tuned_config(int dev, const Geom &g, HybridCfg *cfg);
+
+// Rows per lane M and workgroups per CU of the ragged persistent kernel for
+// systems up to max_len long.
+template <typename REAL>
+void ragged_tuned_config(int dev, int64_t max_len, int *M, int *per_cu);
+
 }  // namespace trid
