@@ -2,15 +2,21 @@
 solve on a 512^3 grid along x, y and z, on one MI355X, with the HBM roofline of
 the dominant kernel and the reference CPU solver timed on this host's cores.
 
-One step = one solve along x, one along y and one along z of the same
-512^3 FP64 arrays (a, b, c read; d overwritten), i.e. 3 * 512^3 grid points.
+One step = one solve along x, one along y and one along z of the same 512^3
+FP64 arrays (a, b, c read; d overwritten), i.e. 3 * 512^3 grid points.
 Inputs are generated on the GPU before the timed region (diagonally dominant:
-a, c ~ U(-1,1), b = 2 + |a| + |c| + U(0,1), d ~ U(-1,1); fixed seed).
+a, c ~ U(-1,1), b = 2 + |a| + |c| + U(0,1), d ~ U(-1,1); fixed seed). Each
+direction's kernel shape is autotuned before the timed region.
 
 Algorithmic bytes per grid point (the roofline's `achieved`): read a, b, c, d
 and write d = 5 x 8 = 40 B (FP64).
 
-Prints ONE JSON line (the driver's contract) and, with --verbose, a table.
+Extra measurements in the same JSON line ("extras"; skipped by --quick):
+FP32 x/y/z; FP32 sweeps + FP64 refinement vs FP64 (time ratio and error);
+adjoint / forward time on 256^3 FP64; the ragged 40,000-system batch; the
+hipGraph-replayed ADI step on 256^3.
+
+Prints ONE JSON line (the driver's contract).
 """
 from __future__ import annotations
 
@@ -35,14 +41,19 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=512, help="grid edge (n^3 points)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-tune", action="store_true", help="use default kernel shapes")
+    ap.add_argument("--quick", action="store_true", help="headline metric only")
     ap.add_argument("--cpu-slab", type=int, default=64,
                     help="z-extent of the CPU baseline sample (n x n x slab)")
     ap.add_argument("--verbose", action="store_true")
     return ap.parse_args()
 
 
-def gen_gpu(n3, dtype, seed):
-    import torch
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def gen_gpu(torch, n3, dtype, seed):
     g = torch.Generator(device="cuda")
     g.manual_seed(seed)
     a = torch.rand(n3, dtype=dtype, device="cuda", generator=g) * 2 - 1
@@ -52,20 +63,29 @@ def gen_gpu(n3, dtype, seed):
     return a, b, c, d
 
 
+def event_ms(torch, fn, reps=5, warm=1):
+    """Median device time of fn() over `reps`, measured with events on the
+    current stream (the library is bound to it)."""
+    for _ in range(warm):
+        fn()
+    ts = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        e1.synchronize()
+        ts.append(e0.elapsed_time(e1))
+    return statistics.median(ts)
+
+
 def measure_copy_bw(torch, lib, nbytes=4 << 30, reps=10):
     """HBM bandwidth of the library's stream-copy kernel (read + write bytes)."""
     src = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").fill_(1.0)
     dst = torch.empty_like(src)
-    for _ in range(2):
-        lib.tridHipStreamCopy(C.c_void_p(src.data_ptr()), C.c_void_p(dst.data_ptr()), nbytes)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(reps):
-        lib.tridHipStreamCopy(C.c_void_p(src.data_ptr()), C.c_void_p(dst.data_ptr()), nbytes)
-    e1.record()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / reps
+    ms = event_ms(torch, lambda: lib.tridHipStreamCopy(C.c_void_p(src.data_ptr()),
+                                                       C.c_void_p(dst.data_ptr()), nbytes),
+                  reps=reps, warm=2)
     del src, dst
     return 2 * nbytes / (ms * 1e-3) / 1e9
 
@@ -110,7 +130,119 @@ def cpu_baseline(n, slab, reps=3):
                                       n, n, slab, reps, threads,
                                       oracle.ref_isa() if kind == "reference" else "scalar"))
     return {"value": pts_s, "unit": "grid pts/s", "cores": threads, "kind": kind,
-            "sample": desc, "per_dir_s": per_dir}
+            "sample": desc,
+            "per_dir_gpts_s": {k: round(n * n * slab / v / 1e9, 3) for k, v in per_dir.items()}}
+
+
+def bench_fp32_dirs(torch, api, n, tune):
+    n3 = n ** 3
+    a, b, c, d = gen_gpu(torch, n3, torch.float32, seed=32)
+    out = {}
+    for sd, name in enumerate("xyz"):
+        if tune:
+            api.autotune(torch.float32, sd, [n, n, n])
+        ms = event_ms(torch, lambda: api.trid_strided_batch(a, b, c, d, solvedim=sd,
+                                                            dims=[n, n, n]))
+        gbs = 20.0 * n3 / (ms * 1e-3) / 1e9
+        out[name] = {"ms": round(ms, 4), "gpts_s": round(n3 / ms / 1e6, 2),
+                     "alg_GBs": round(gbs, 1), "frac_peak": round(gbs / PEAK_HBM_GBS, 4),
+                     "config": api.last_config()}
+    del a, b, c, d
+    torch.cuda.empty_cache()
+    return out
+
+
+def bench_mixed(torch, api, n, fp64_ms, tune):
+    n3 = n ** 3
+    a, b, c, d0 = gen_gpu(torch, n3, torch.float64, seed=64)
+    out = {}
+    for sd, name in enumerate("xyz"):
+        if tune:
+            api.autotune_mixed(sd, [n, n, n])
+        x64 = d0.clone()
+        api.trid_strided_batch(a, b, c, x64, solvedim=sd, dims=[n, n, n])
+        xm = d0.clone()
+        api.mixed_strided_batch(a, b, c, xm, solvedim=sd, dims=[n, n, n], refinements=2)
+        err = float((xm - x64).abs().max() / x64.abs().max())
+        del xm, x64
+        d = d0.clone()
+        ms = event_ms(torch, lambda: api.mixed_strided_batch(a, b, c, d, solvedim=sd,
+                                                             dims=[n, n, n], refinements=2))
+        out[name] = {"ms": round(ms, 4), "gpts_s": round(n3 / ms / 1e6, 2),
+                     "speedup_vs_fp64": round(fp64_ms[name] / ms, 3),
+                     "max_rel_err_vs_fp64": err, "config": api.last_config()}
+        del d
+    del a, b, c, d0
+    torch.cuda.empty_cache()
+    return out
+
+
+def bench_adjoint(torch, api, n=256):
+    n3 = n ** 3
+    dims = [n, n, n]
+    a, b, c, d = gen_gpu(torch, n3, torch.float64, seed=256)
+    w = torch.randn(n3, dtype=torch.float64, device="cuda")
+    out = {}
+    for sd, name in enumerate("xyz"):
+        x = d.clone()
+        fwd = event_ms(torch, lambda: api.trid_strided_batch(a, b, c, x, solvedim=sd, dims=dims))
+        x = d.clone()
+        api.trid_strided_batch(a, b, c, x, solvedim=sd, dims=dims)
+        lam, ga, gb, gc = (torch.empty_like(x) for _ in range(4))
+        adj_all = event_ms(torch, lambda: api.adjoint_strided_batch(
+            a, b, c, x, w, solvedim=sd, dims=dims, lam=lam, grad_a=ga, grad_b=gb, grad_c=gc))
+        adj_d = event_ms(torch, lambda: api.adjoint_strided_batch(
+            a, b, c, x, w, solvedim=sd, dims=dims, lam=lam))
+        out[name] = {"fwd_ms": round(fwd, 4), "adj_all_ms": round(adj_all, 4),
+                     "adj_dL_dd_ms": round(adj_d, 4),
+                     "adj_over_fwd_all_grads": round(adj_all / fwd, 3),
+                     "adj_over_fwd_dL_dd_only": round(adj_d / fwd, 3),
+                     "byte_floor_all_grads": 72 / 40, "byte_floor_dL_dd_only": 40 / 40}
+    del a, b, c, d, w
+    torch.cuda.empty_cache()
+    return out
+
+
+def bench_ragged(torch, api, tune):
+    import numpy as np
+    rng = np.random.default_rng(40000)
+    lens = np.round(np.exp(rng.uniform(np.log(8), np.log(8192), 40000))).astype(np.int64)
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    total = int(off[-1])
+    toff = torch.from_numpy(off).cuda()
+    out = {"systems": 40000, "unknowns": total}
+    for dt, key, bpp in ((torch.float64, "fp64", 40.0), (torch.float32, "fp32", 20.0)):
+        a, b, c, d = gen_gpu(torch, total, dt, seed=8192)
+        if tune:
+            api.autotune_ragged(dt, toff, 8192)
+        ms = event_ms(torch, lambda: api.ragged_batch(a, b, c, d, toff, max_len=8192))
+        gbs = (bpp * total + 8 * (len(off))) / (ms * 1e-3) / 1e9
+        out[key] = {"ms": round(ms, 4), "gpts_s": round(total / ms / 1e6, 2),
+                    "alg_GBs": round(gbs, 1), "frac_peak": round(gbs / PEAK_HBM_GBS, 4),
+                    "config": api.last_config()}
+        del a, b, c, d
+    torch.cuda.empty_cache()
+    return out
+
+
+def bench_adi(torch, n=256, iters=20):
+    from tridsolver_amd import adi
+    out = {}
+    for dt, key in ((torch.float64, "fp64"), (torch.float32, "fp32")):
+        u = adi.initial_u(n, n, n, dt, "random", seed=1)
+        ws = adi.Workspace(n, n, n, dt)
+        adi.run(1.0, u, ws, 2, graph=True, sync=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        adi.run(1.0, u, ws, iters, graph=True, sync=True)
+        dt_s = time.perf_counter() - t0
+        step_ms = dt_s * 1e3 / iters
+        out[key] = {"step_ms": round(step_ms, 4), "gpts_s": round(n ** 3 / step_ms / 1e6, 2),
+                    "steps": iters,
+                    "bytes_per_point_step": 168 if key == "fp64" else 84}
+        del u, ws
+    torch.cuda.empty_cache()
+    return out
 
 
 def main():
@@ -128,14 +260,18 @@ def main():
     n = args.n
     n3 = n ** 3
     dims = [n, n, n]
-    dt = torch.float64
     bytes_per_pt = 40.0
+    tune = not args.no_tune
 
-    copy_bw = measure_copy_bw(torch, lib)
-
-    a, b, c, d = gen_gpu(n3, dt, seed=512)
-    torch.cuda.synchronize()
     api.use_torch_stream()  # library launches on torch's current stream
+    copy_bw = measure_copy_bw(torch, lib)
+    log("copy bandwidth %.1f GB/s" % copy_bw)
+
+    if tune:
+        for sd in range(3):
+            api.autotune(torch.float64, sd, dims)
+    a, b, c, d = gen_gpu(torch, n3, torch.float64, seed=512)
+    torch.cuda.synchronize()
 
     def solve(sd):
         api.trid_strided_batch(a, b, c, d, solvedim=sd, dims=dims, sync=False)
@@ -165,6 +301,8 @@ def main():
                                                 for k in range(args.steps))
     ms_per_step = wall * 1e3 / args.steps
     value = 3 * n3 * args.steps / wall
+    del a, b, c, d
+    torch.cuda.empty_cache()
 
     dirs = {}
     for name, ms in per_dir_ms.items():
@@ -172,17 +310,32 @@ def main():
         dirs[name] = {"ms": round(ms, 4), "gpts_s": round(n3 / (ms * 1e-3) / 1e9, 3),
                       "alg_GBs": round(gbs, 1), "frac_peak": round(gbs / PEAK_HBM_GBS, 4),
                       "frac_copy": round(gbs / copy_bw, 4), "config": cfgs[name]}
+        log("fp64 %s: %s" % (name, dirs[name]))
     dom = max(per_dir_ms, key=per_dir_ms.get)
     ach = bytes_per_pt * n3 / (per_dir_ms[dom] * 1e-3) / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("fp64_512_" + dom)
+            traffic = json.load(f).get("fp64_%d_%s" % (n, dom))
+
+    extras = {}
+    if not args.quick:
+        extras["fp32_per_direction"] = bench_fp32_dirs(torch, api, n, tune)
+        log("fp32: %s" % extras["fp32_per_direction"])
+        extras["fp32_refined_vs_fp64"] = bench_mixed(torch, api, n, per_dir_ms, tune)
+        log("mixed: %s" % extras["fp32_refined_vs_fp64"])
+        extras["adjoint_256"] = bench_adjoint(torch, api)
+        log("adjoint: %s" % extras["adjoint_256"])
+        extras["ragged_40k"] = bench_ragged(torch, api, tune)
+        log("ragged: %s" % extras["ragged_40k"])
+        extras["adi_256_graph"] = bench_adi(torch)
+        log("adi: %s" % extras["adi_256_graph"])
 
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(n, args.cpu_slab)
+        log("cpu: %s" % cpu)
 
     out = {
         "metric": "grid pts solved/s (FP64 512^3 x/y/z batched tridiagonal solves)",
@@ -197,22 +350,18 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded diagonally dominant a,b,c,d generated on the GPU)",
-        "config": {"workload": "512^3 FP64 strided batch, solve along x, y, z (one step = "
-                               "3 solves of 134,217,728 unknowns)",
-                   "grid": dims, "bytes_per_point": bytes_per_pt},
+        "config": {"workload": "%d^3 FP64 strided batch, solve along x, y, z (one step = "
+                               "3 solves of %d unknowns)" % (n, n3),
+                   "grid": dims, "bytes_per_point": bytes_per_pt, "autotuned": tune},
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(ach / PEAK_HBM_GBS, 4),
-                     "traffic": traffic, "kernel": "solvedim %s" % dom},
+                     "traffic": traffic, "kernel": "solvedim %s (slowest direction)" % dom},
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in
                                                   ("value", "unit", "cores", "kind", "sample")},
         "per_direction": dirs,
         "copy_GBs": round(copy_bw, 1),
+        "extras": extras,
     }
-    if args.verbose:
-        for name, r in dirs.items():
-            print("%s: %s" % (name, r), file=sys.stderr)
-        if cpu:
-            print("cpu: %s" % cpu, file=sys.stderr)
     print(json.dumps(out))
 
 

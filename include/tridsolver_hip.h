@@ -54,6 +54,37 @@ EXTERN_C tridStatus_t tridHipLastConfig(int *algorithm, int *M, int *P, int *W, 
  * makes that call return TRID_STATUS_INVALID_VALUE. */
 EXTERN_C tridStatus_t tridHipForceHybridConfig(int M, int P, int W, int G);
 
+/* ---- autotuner -------------------------------------------------------------
+ * No counterpart in the reference, whose kernel shapes are compile-time tables
+ * (scalar/src/cuda/trid_thomaspcr.hpp:521-601). The shape of each call -- the
+ * per-lane Thomas kernel, or the hybrid's rows per lane M (the Thomas / PCR
+ * split), systems per workgroup W and XCD grouping G -- comes from a
+ * per-device cache keyed by the exact geometry; uncached geometries use
+ * measured defaults. Tuning times every valid shape on scratch arrays of the
+ * same geometry (the caller's data is never touched) and caches the fastest.
+ * Tuning allocates device memory and synchronises: call it outside hipGraph
+ * capture. */
+
+/* mode 0: never tune implicitly (default; environment TRIDSOLVER_AUTOTUNE=1
+ * selects mode 1 at start-up); mode 1: tune every new geometry on its first
+ * call outside stream capture. */
+EXTERN_C tridStatus_t tridHipSetAutotuneMode(int mode);
+/* Tunes the batched solve (fp64: 1 double / 0 float; inc: the Inc variant)
+ * for one geometry; *best_ms (may be NULL) receives the winner's time. */
+EXTERN_C tridStatus_t tridHipAutotune(int fp64, int inc, int ndim, int solvedim,
+                                      const int *dims, const int *pads, float *best_ms);
+/* Tunes tridDmtsvStridedBatchMixed (2 refinements) for one geometry. */
+EXTERN_C tridStatus_t tridHipAutotuneMixed(int ndim, int solvedim, const int *dims,
+                                           const int *pads, float *best_ms);
+/* Tunes the ragged kernel (rows per lane, workgroups per CU) on the caller's
+ * device offsets with scratch values; cached per (precision, max_len). */
+EXTERN_C tridStatus_t tridHipAutotuneRagged(int fp64, const int64_t *offsets, int64_t nsys,
+                                            int64_t max_len);
+/* Saves / loads the current device's cache as text; clears all entries. */
+EXTERN_C tridStatus_t tridHipAutotuneSave(const char *path);
+EXTERN_C tridStatus_t tridHipAutotuneLoad(const char *path);
+EXTERN_C tridStatus_t tridHipAutotuneClear(void);
+
 /* ---- FP32 sweeps with FP64 refinement ------------------------------------
  * No counterpart in the reference. Same arguments and layout as
  * tridDmtsvStridedBatch (scalar/include/tridsolver.h:38-42) plus the number

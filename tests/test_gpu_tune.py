@@ -1,0 +1,91 @@
+"""Autotuner: tuned shapes are used by later calls, results stay correct
+(normwise <= 1e-12 FP64 vs the oracle), the cache survives save/load, and
+tune-on-first-use mode tunes new geometries."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from tests import helpers as H
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+from tridsolver_amd import _lib, api  # noqa: E402
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    torch.cuda.set_device(0)
+    yield
+    _lib.lib().tridHipAutotuneClear()
+    _lib.lib().tridHipSetAutotuneMode(0)
+
+
+def dev(x):
+    return torch.from_numpy(np.ascontiguousarray(x)).cuda()
+
+
+def solve_and_check(dims, sd, dtype=np.float64, seed=0):
+    rng = np.random.default_rng(seed)
+    n = int(np.prod(dims))
+    a, b, c, d = H.diag_dominant(rng, n, dtype)
+    td = dev(d)
+    api.trid_strided_batch(dev(a), dev(b), dev(c), td, solvedim=sd, dims=dims, sync=True)
+    od = d.copy()
+    oracle.batch(a, b, c, od, ndim=len(dims), solvedim=sd, dims=dims)
+    H.assert_norm_close(td.cpu().numpy(), od, dtype)
+    return api.last_config()
+
+
+@pytest.mark.parametrize("sd", [0, 1, 2])
+def test_autotune_then_solve(sd, tmp_path):
+    dims = [96, 80, 72]
+    _lib.lib().tridHipAutotuneClear()
+    ms = api.autotune(torch.float64, sd, dims)
+    assert ms > 0
+    cfg = solve_and_check(dims, sd)
+    path = str(tmp_path / "tune.txt").encode()
+    assert _lib.lib().tridHipAutotuneSave(path) == 0
+    assert _lib.lib().tridHipAutotuneClear() == 0
+    assert _lib.lib().tridHipAutotuneLoad(path) == 0
+    assert solve_and_check(dims, sd) == cfg
+    with open(path.decode()) as f:
+        assert len([ln for ln in f if not ln.startswith("#")]) >= 1
+
+
+def test_autotune_on_first_use():
+    _lib.lib().tridHipAutotuneClear()
+    assert _lib.lib().tridHipSetAutotuneMode(1) == 0
+    try:
+        for sd in range(3):
+            solve_and_check([50, 44, 38], sd, np.float32, seed=sd)
+    finally:
+        _lib.lib().tridHipSetAutotuneMode(0)
+
+
+def test_autotune_mixed_and_ragged():
+    dims = [128, 64, 32]
+    for sd in range(3):
+        assert api.autotune_mixed(sd, dims) > 0
+        rng = np.random.default_rng(sd)
+        n = int(np.prod(dims))
+        a, b, c, d = H.diag_dominant(rng, n)
+        td = dev(d)
+        api.mixed_strided_batch(dev(a), dev(b), dev(c), td, solvedim=sd, dims=dims, sync=True)
+        xl = oracle.batch_ld(a, b, c, d, ndim=3, solvedim=sd, dims=dims)
+        assert H.norm_rel_err(td.cpu().numpy(), xl) <= 1e-12
+    rng = np.random.default_rng(12)
+    lens = np.round(np.exp(rng.uniform(np.log(8), np.log(4096), 2000))).astype(np.int64)
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    a, b, c, d = H.diag_dominant(rng, int(off[-1]))
+    toff = dev(off)
+    api.autotune_ragged(torch.float64, toff, 4096)
+    td = dev(d)
+    api.ragged_batch(dev(a), dev(b), dev(c), td, toff, max_len=4096, sync=True)
+    want = d.copy()
+    oracle.ragged(a, b, c, want, off)
+    H.assert_norm_close(td.cpu().numpy(), want, np.float64)

@@ -72,6 +72,13 @@ SIGNATURES = {
     "tridDmtsvRaggedBatchInc": ([P] * 7 + [L, L], I),
     "tridSmtsvRaggedBatchInc": ([P] * 7 + [L, L], I),
     "tridHipRaggedWorkspaceBytes": ([L], SZ),
+    "tridHipSetAutotuneMode": ([I], I),
+    "tridHipAutotune": ([I, I, I, I, P, P, C.POINTER(C.c_float)], I),
+    "tridHipAutotuneMixed": ([I, I, P, P, C.POINTER(C.c_float)], I),
+    "tridHipAutotuneRagged": ([I, P, L, L], I),
+    "tridHipAutotuneSave": ([C.c_char_p], I),
+    "tridHipAutotuneLoad": ([C.c_char_p], I),
+    "tridHipAutotuneClear": ([], I),
     "tridHipForceRaggedConfig": ([I, I], I),
     # include/tridsolver_adi.h
     "tridAdiPreprocD": ([D] + [P] * 11 + [I, I, I], I),

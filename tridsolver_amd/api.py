@@ -190,3 +190,28 @@ class forced_ragged:
     def __exit__(self, *exc):
         _lib.lib().tridHipForceRaggedConfig(0, 1)
         return False
+
+
+def autotune(dtype, solvedim, dims, pads=None, inc=False):
+    """Tunes the solve for one geometry (scratch data); returns the best time
+    in ms. Later solves of that geometry use the winning shape."""
+    pads = dims if pads is None else pads
+    ms = C.c_float(0)
+    st = _lib.lib().tridHipAutotune(1 if dtype == torch.float64 else 0, 1 if inc else 0,
+                                    len(dims), solvedim, int3(dims), int3(pads), C.byref(ms))
+    check(st, "tridHipAutotune")
+    return ms.value
+
+
+def autotune_mixed(solvedim, dims, pads=None):
+    pads = dims if pads is None else pads
+    ms = C.c_float(0)
+    check(_lib.lib().tridHipAutotuneMixed(len(dims), solvedim, int3(dims), int3(pads),
+                                          C.byref(ms)), "tridHipAutotuneMixed")
+    return ms.value
+
+
+def autotune_ragged(dtype, offsets, max_len):
+    check(_lib.lib().tridHipAutotuneRagged(1 if dtype == torch.float64 else 0, _ptr(offsets),
+                                           offsets.numel() - 1, int(max_len)),
+          "tridHipAutotuneRagged")
