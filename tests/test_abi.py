@@ -93,7 +93,8 @@ def test_workspace_bytes():
 
 def test_force_hybrid_config_validation():
     lib = _lib.lib()
-    assert lib.tridHipForceHybridConfig(-1, 1, 1, 1) == 3
-    assert lib.tridHipForceHybridConfig(8, 0, 1, 1) == 3
-    assert lib.tridHipForceHybridConfig(8, 1, 1, 0) == 3
-    assert lib.tridHipForceHybridConfig(0, 0, 0, 0) == 0
+    assert lib.tridHipForceHybridConfig(-1, 1, 1, 1, 0) == 3
+    assert lib.tridHipForceHybridConfig(8, 0, 1, 1, 0) == 3
+    assert lib.tridHipForceHybridConfig(8, 1, 1, 0, 0) == 3
+    assert lib.tridHipForceHybridConfig(8, 1, 1, 1, 2) == 3
+    assert lib.tridHipForceHybridConfig(0, 0, 0, 0, 0) == 0

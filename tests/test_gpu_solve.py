@@ -135,6 +135,24 @@ def test_forced_hybrid_shapes(M, W, G, dtype):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("W,G", [(1, 1), (16, 8), (32, 3), (5, 16)])
+@pytest.mark.parametrize("inc", [False, True])
+def test_forced_streamed_variant(W, G, inc, dtype):
+    """The streamed strided variant (loads a few rows ahead of the sweep):
+    every compiled M, y and z, lengths not multiples of M, Inc."""
+    Ms = (8, 16) if dtype == np.float64 else (16, 32)
+    for M in Ms:
+        for dims, sd in (([19, 45, 3], 1), ([17, 4, 51], 2), ([40, 37, 9], 1), ([33, 8, 130], 2)):
+            N = dims[sd]
+            P = -(-N // M)
+            if W * P > 1024:
+                continue
+            with api.forced_hybrid(M, P, W, G, 1):
+                cfg = run_case(dims, sd, dtype, inc, (0, 0, 0), seed=M * 13 + W)
+            assert cfg == (6, M, P, W, G)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_edge_lengths(dtype):
     for dims in ([1], [2], [1, 9], [9, 1], [2, 3, 1], [3000], [2500, 2]):
         for sd in range(len(dims)):

@@ -62,7 +62,7 @@ SIGNATURES = {
     "tridHipReleaseWorkspaces": ([], I),
     "tridHipWorkspaceBytes": ([I, I, I, I, P], SZ),
     "tridHipLastConfig": ([IP, IP, IP, IP, IP], I),
-    "tridHipForceHybridConfig": ([I, I, I, I], I),
+    "tridHipForceHybridConfig": ([I, I, I, I, I], I),
     "tridHipStreamCopy": ([P, P, SZ], I),
     "tridDmtsvStridedBatchMixed": ([P, P, P, P, P, I, I, P, P, I], I),
     "tridDmtsvStridedBatchAdjoint": ([P] * 10 + [I, I, P, P], I),

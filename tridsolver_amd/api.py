@@ -101,15 +101,15 @@ class forced_hybrid:
     """Context manager forcing the hybrid shape (M rows/lane, P lanes/system,
     W systems/workgroup, G XCD grouping) for this thread's solves."""
 
-    def __init__(self, M, P, W, G=8):
-        self.cfg = (M, P, W, G)
+    def __init__(self, M, P, W, G=8, variant=0):
+        self.cfg = (M, P, W, G, variant)
 
     def __enter__(self):
         check(_lib.lib().tridHipForceHybridConfig(*self.cfg), "tridHipForceHybridConfig")
         return self
 
     def __exit__(self, *exc):
-        _lib.lib().tridHipForceHybridConfig(0, 0, 0, 0)
+        _lib.lib().tridHipForceHybridConfig(0, 0, 0, 0, 0)
         return False
 
 

@@ -1,0 +1,155 @@
+// kernels_stream.hip - strided Thomas-PCR hybrid with streamed loads, for wide
+// row segments (256 bytes: 32 FP64 or 64 FP32 x-neighbouring systems per
+// workgroup).
+//
+// The plain hybrid (kernels_solve.hip) issues all 4*M loads of a lane before
+// its forward sweep, so it needs registers for a, b, c and d of every row;
+// 32 FP64 systems of 512 rows would not fit the register file. Here the loads
+// run LA rows ahead of the forward sweep (a ring of LA rows) and scheduling
+// barriers stop the compiler from hoisting them all to the top, so the live
+// set is the sweep's three values per row (al, ga, de) plus the ring. The
+// arithmetic is exactly the hybrid's (hybrid_core.hpp phases (i)-(iv)); only
+// the order of loads differs.
+//
+// Why wider segments: a workgroup reads each row of its systems as one
+// contiguous segment of W * sizeof(REAL) bytes. For z solves every row lies in
+// a different 2 MiB page and for y in a different DRAM row, so fewer, wider
+// segments mean fewer translations and activations per byte (a 128-byte
+// segment streams at ~4.0 TB/s for z, a 256-byte one at ~5.0 TB/s).
+
+#include <algorithm>
+#include <atomic>
+
+#include "hybrid_core.hpp"
+#include "trid_internal.h"
+
+namespace trid {
+namespace {
+
+constexpr int STREAM_THREADS = 1024;
+
+template <typename REAL, int M, int LA, bool INC>
+__global__ __launch_bounds__(STREAM_THREADS) void hybrid_stream_kernel(
+    Geom g, int P, int W, int G, const REAL *__restrict__ a, const REAL *__restrict__ b,
+    const REAL *__restrict__ c, REAL *__restrict__ d, REAL *__restrict__ u) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const hyb::Lane L = hyb::lane_of<false>(g, P, W, G);
+  hyb::Ring ring{smem_raw, (int)(3 * sizeof(REAL) * L.T), 0};
+  const int64_t row0 = (int64_t)L.p * M;
+
+  auto load_row = [&](int r, REAL &ar, REAL &br, REAL &cr, REAL &dr) {
+    const int64_t row = row0 + r;
+    const bool ok = L.active && row < L.N;
+    const int64_t o = L.base + row * L.stride;
+    ar = (ok && row > 0) ? a[o] : REAL(0);
+    br = ok ? b[o] : REAL(1);
+    cr = (ok && row < L.N - 1) ? c[o] : REAL(0);
+    dr = ok ? d[o] : REAL(0);
+  };
+
+  REAL al[M], ga[M], de[M];
+  REAL qa[LA], qb[LA], qc[LA], qd[LA];
+#pragma unroll
+  for (int k = 0; k < LA; ++k) load_row(k, qa[k], qb[k], qc[k], qd[k]);
+#pragma unroll
+  for (int r = 0; r < M; ++r) {
+    const int s = r % LA;
+    const REAL ar = qa[s], br = qb[s], cr = qc[s], dr = qd[s];
+    if (r + LA < M) load_row(r + LA, qa[s], qb[s], qc[s], qd[s]);
+    // keep the next row's loads here, behind this row's arithmetic
+    __builtin_amdgcn_sched_barrier(0);
+    if (r == 0) {
+      const REAL inv = REAL(1) / br;
+      al[0] = ar * inv;
+      ga[0] = cr * inv;
+      de[0] = dr * inv;
+    } else {
+      const REAL inv = REAL(1) / (br - ar * ga[r - 1]);
+      al[r] = -ar * al[r - 1] * inv;
+      ga[r] = cr * inv;
+      de[r] = (dr - ar * de[r - 1]) * inv;
+    }
+  }
+  const REAL al_e = al[M - 1], ga_e = ga[M - 1], de_e = de[M - 1];
+  REAL f_d, f_a, f_g;
+  hyb::backward_spike<REAL, M>(al, ga, de, f_d, f_a, f_g);
+  REAL y, ym;
+  hyb::reduced_solve<REAL>(L, ring, al_e, ga_e, de_e, f_d, f_a, f_g, y, ym);
+  if (!L.active) return;
+#pragma unroll
+  for (int r = 0; r < M; ++r) {
+    const int64_t row = row0 + r;
+    if (row < L.N) {
+      const REAL x = (r < M - 1) ? de[r] - al[r] * ym - ga[r] * y : y;
+      const int64_t o = L.base + row * L.stride;
+      if (INC) u[o] += x;
+      else d[o] = x;
+    }
+  }
+}
+
+// Lookahead rows: as deep as the register file allows at 1024 threads
+// (FP64 M = 16 and FP32 M = 32 hold 96 VGPRs of sweep state).
+template <typename REAL, int M>
+constexpr int stream_lookahead() {
+  return sizeof(REAL) * M >= 128 ? 2 : 4;
+}
+
+template <typename REAL, int M, bool INC>
+hipError_t launch_stream_m(const Geom &g, const HybridCfg &cfg, const REAL *a, const REAL *b,
+                           const REAL *c, REAL *d, REAL *u, hipStream_t st) {
+  constexpr int LA = stream_lookahead<REAL, M>();
+  const int T = cfg.W * cfg.P;
+  const size_t lds = hyb::ring_bytes<REAL>(T, 3);
+  auto kern = hybrid_stream_kernel<REAL, M, LA, INC>;
+  static std::atomic<size_t> lds_granted{0};
+  {
+    hipError_t e = ensure_lds(reinterpret_cast<const void *>(kern), lds, lds_granted);
+    if (e != hipSuccess) return e;
+  }
+  const int64_t blocks = (g.n0 + cfg.W - 1) / cfg.W * g.n1;
+  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(T), lds, st, g, cfg.P, cfg.W, cfg.G, a,
+                     b, c, d, u);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+// Rows per lane compiled for the streamed variant.
+bool stream_m_supported(int fp64, int M) { return fp64 ? (M == 8 || M == 16) : (M == 16 || M == 32); }
+
+template <typename REAL>
+bool stream_cfg_valid(const Geom &g, const HybridCfg &c) {
+  if (g.stride == 1) return false;  // strided systems only
+  if (!stream_m_supported(sizeof(REAL) == 8, c.M)) return false;
+  if (c.P < 1 || c.W < 1 || c.G < 1) return false;
+  if ((int64_t)c.P * c.M < g.N) return false;
+  const int64_t T = (int64_t)c.W * c.P;
+  return T <= STREAM_THREADS && hyb::ring_bytes<REAL>((int)T, 3) <= 160 * 1024;
+}
+
+template <typename REAL, bool INC>
+hipError_t launch_stream(const Geom &g, const HybridCfg &cfg, const REAL *a, const REAL *b,
+                         const REAL *c, REAL *d, REAL *u, hipStream_t st) {
+  if constexpr (sizeof(REAL) == 8) {
+    if (cfg.M == 8) return launch_stream_m<REAL, 8, INC>(g, cfg, a, b, c, d, u, st);
+    if (cfg.M == 16) return launch_stream_m<REAL, 16, INC>(g, cfg, a, b, c, d, u, st);
+  } else {
+    if (cfg.M == 16) return launch_stream_m<REAL, 16, INC>(g, cfg, a, b, c, d, u, st);
+    if (cfg.M == 32) return launch_stream_m<REAL, 32, INC>(g, cfg, a, b, c, d, u, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+#define TRID_STREAM_INST(REAL)                                                               \
+  template bool stream_cfg_valid<REAL>(const Geom &, const HybridCfg &);                      \
+  template hipError_t launch_stream<REAL, false>(const Geom &, const HybridCfg &, const REAL *, \
+                                                 const REAL *, const REAL *, REAL *, REAL *,   \
+                                                 hipStream_t);                                \
+  template hipError_t launch_stream<REAL, true>(const Geom &, const HybridCfg &, const REAL *,  \
+                                                const REAL *, const REAL *, REAL *, REAL *,    \
+                                                hipStream_t);
+TRID_STREAM_INST(float)
+TRID_STREAM_INST(double)
+
+}  // namespace trid

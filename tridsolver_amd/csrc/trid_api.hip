@@ -28,7 +28,7 @@ namespace {
 
 thread_local hipStream_t tl_stream = nullptr;
 thread_local int tl_last_algo = 0, tl_last_m = 0, tl_last_p = 0, tl_last_w = 0,
-                 tl_last_g = 0;
+                 tl_last_g = 0, tl_last_v = 0;
 thread_local HybridCfg tl_forced;
 thread_local hipError_t tl_hip_error = hipSuccess;
 
@@ -146,6 +146,7 @@ void record_config(int algo, const HybridCfg *cfg) {
   tl_last_p = cfg ? cfg->P : 0;
   tl_last_w = cfg ? cfg->W : 0;
   tl_last_g = cfg ? cfg->G : 0;
+  tl_last_v = cfg ? cfg->V : 0;
 }
 
 const HybridCfg &forced_config() { return tl_forced; }
@@ -282,7 +283,7 @@ EXTERN_C size_t tridHipWorkspaceBytes(int fp64, int inc, int ndim, int solvedim,
 }
 
 EXTERN_C tridStatus_t tridHipLastConfig(int *algorithm, int *M, int *P, int *W, int *G) {
-  if (algorithm) *algorithm = tl_last_algo;
+  if (algorithm) *algorithm = tl_last_algo == 3 && tl_last_v == 1 ? 6 : tl_last_algo;
   if (M) *M = tl_last_m;
   if (P) *P = tl_last_p;
   if (W) *W = tl_last_w;
@@ -290,15 +291,17 @@ EXTERN_C tridStatus_t tridHipLastConfig(int *algorithm, int *M, int *P, int *W, 
   return TRID_STATUS_SUCCESS;
 }
 
-EXTERN_C tridStatus_t tridHipForceHybridConfig(int M, int P, int W, int G) {
+EXTERN_C tridStatus_t tridHipForceHybridConfig(int M, int P, int W, int G, int variant) {
   if (M == 0) {
     tl_forced = HybridCfg();
     return TRID_STATUS_SUCCESS;
   }
-  if (M < 0 || P < 1 || W < 1 || G < 1) return TRID_STATUS_INVALID_VALUE;
+  if (M < 0 || P < 1 || W < 1 || G < 1 || variant < 0 || variant > 1)
+    return TRID_STATUS_INVALID_VALUE;
   tl_forced.M = M;
   tl_forced.P = P;
   tl_forced.W = W;
   tl_forced.G = G;
+  tl_forced.V = variant;
   return TRID_STATUS_SUCCESS;
 }

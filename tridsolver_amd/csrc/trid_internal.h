@@ -33,11 +33,15 @@ struct Geom {
 // of groups the workgroups are dealt into so that consecutive system groups
 // run on the same XCD at the same time (workgroups are dispatched to the 8
 // XCDs round-robin; G = 1 keeps the plain order).
+// V selects the kernel variant: 0 the plain hybrid (all loads before the
+// sweep), 1 the streamed strided variant (loads a few rows ahead of the sweep,
+// for 256-byte row segments; kernels_stream.hip).
 struct HybridCfg {
   int M = 0;
   int P = 0;
   int W = 0;
   int G = 8;
+  int V = 0;
 };
 
 enum class Algo : int { Auto = 0, Lane = 1, Hybrid = 3 };
@@ -60,6 +64,13 @@ hipError_t launch_hybrid(const Geom &g, const HybridCfg &cfg, const REAL *a,
 hipStream_t current_stream();
 tridStatus_t check_device(int *dev);
 tridStatus_t get_workspace(int dev, hipStream_t st, size_t bytes, void **out);
+
+// ---- streamed strided variant (kernels_stream.hip) ------------------------
+template <typename REAL>
+bool stream_cfg_valid(const Geom &g, const HybridCfg &c);
+template <typename REAL, bool INC>
+hipError_t launch_stream(const Geom &g, const HybridCfg &cfg, const REAL *a, const REAL *b,
+                         const REAL *c, REAL *d, REAL *u, hipStream_t st);
 
 // ---- mixed precision (kernels_mixed.hip), FP64 operands ------------------
 bool mixed_cfg_valid(const Geom &g, const HybridCfg &c);

@@ -162,7 +162,25 @@ std::vector<HybridCfg> candidates(const Geom &g) {
         if ((int64_t)W > g.n0 && W > 1) continue;
         if ((int64_t)c.W * c.P < 32 && c.W < g.n0) continue;  // under-filled workgroups
         if (hybrid_cfg_valid<REAL>(g, c)) out.push_back(c);
+        if (g.stride != 1) {  // streamed strided variant of the same shape
+          c.V = 1;
+          if (hybrid_cfg_valid<REAL>(g, c)) out.push_back(c);
+        }
       }
+  if (g.stride != 1) {  // the streamed variant also runs M = 32 (FP32)
+    for (int W : Ws)
+      for (int G : Gs) {
+        HybridCfg c;
+        c.M = 32;
+        c.P = (int)((g.N + 31) / 32);
+        c.W = W;
+        c.G = G;
+        c.V = 1;
+        if ((int64_t)W * (int64_t)sizeof(REAL) < 64 && W < g.n0) continue;
+        if ((int64_t)W > g.n0 && W > 1) continue;
+        if (hybrid_cfg_valid<REAL>(g, c)) out.push_back(c);
+      }
+  }
   return out;
 }
 
@@ -420,7 +438,7 @@ EXTERN_C tridStatus_t tridHipAutotuneRagged(int fp64, const int64_t *offsets, in
 }
 
 // Text format, one entry per line:
-//   kind fp64 inc N stride n0 n1 st0 st1 algo M P W G ms
+//   kind fp64 inc N stride n0 n1 st0 st1 algo M P W G V ms
 // (the device is not stored: entries load onto the current device).
 EXTERN_C tridStatus_t tridHipAutotuneSave(const char *path) {
   if (!path) return TRID_STATUS_INVALID_VALUE;
@@ -429,15 +447,15 @@ EXTERN_C tridStatus_t tridHipAutotuneSave(const char *path) {
   FILE *f = std::fopen(path, "w");
   if (!f) return TRID_STATUS_INVALID_VALUE;
   std::lock_guard<std::mutex> lk(g_tmu);
-  std::fprintf(f, "# tridsolver_amd autotune cache v1\n");
+  std::fprintf(f, "# tridsolver_amd autotune cache v2\n");
   for (const auto &kv : g_cache) {
     const Key &k = kv.first;
     const Choice &c = kv.second;
     if (k.dev != dev) continue;
-    std::fprintf(f, "%d %d %d %lld %lld %lld %lld %lld %lld %d %d %d %d %d %.6f\n", k.kind,
+    std::fprintf(f, "%d %d %d %lld %lld %lld %lld %lld %lld %d %d %d %d %d %d %.6f\n", k.kind,
                  k.fp64, k.inc, (long long)k.N, (long long)k.stride, (long long)k.n0,
                  (long long)k.n1, (long long)k.st0, (long long)k.st1, c.algo, c.cfg.M, c.cfg.P,
-                 c.cfg.W, c.cfg.G, c.ms);
+                 c.cfg.W, c.cfg.G, c.cfg.V, c.ms);
   }
   std::fclose(f);
   return TRID_STATUS_SUCCESS;
@@ -456,9 +474,9 @@ EXTERN_C tridStatus_t tridHipAutotuneLoad(const char *path) {
     Key k{};
     Choice c{};
     long long N, stride, n0, n1, st0, st1;
-    if (std::sscanf(line, "%d %d %d %lld %lld %lld %lld %lld %lld %d %d %d %d %d %f", &k.kind,
+    if (std::sscanf(line, "%d %d %d %lld %lld %lld %lld %lld %lld %d %d %d %d %d %d %f", &k.kind,
                     &k.fp64, &k.inc, &N, &stride, &n0, &n1, &st0, &st1, &c.algo, &c.cfg.M,
-                    &c.cfg.P, &c.cfg.W, &c.cfg.G, &c.ms) != 15) {
+                    &c.cfg.P, &c.cfg.W, &c.cfg.G, &c.cfg.V, &c.ms) != 16) {
       s = TRID_STATUS_INVALID_VALUE;
       break;
     }
