@@ -137,9 +137,11 @@ def test_forced_hybrid_shapes(M, W, G, dtype):
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("W,G", [(1, 1), (16, 8), (32, 3), (5, 16)])
 @pytest.mark.parametrize("inc", [False, True])
-def test_forced_streamed_variant(W, G, inc, dtype):
-    """The streamed strided variant (loads a few rows ahead of the sweep):
-    every compiled M, y and z, lengths not multiples of M, Inc."""
+@pytest.mark.parametrize("variant", [1, 2])
+def test_forced_streamed_variant(W, G, inc, variant, dtype):
+    """The streamed strided variants (loads a few rows ahead of the sweep; 2:
+    one sweep value per row in LDS): every compiled M, y and z, lengths not
+    multiples of M, Inc."""
     Ms = (8, 16) if dtype == np.float64 else (16, 32)
     for M in Ms:
         for dims, sd in (([19, 45, 3], 1), ([17, 4, 51], 2), ([40, 37, 9], 1), ([33, 8, 130], 2)):
@@ -147,9 +149,9 @@ def test_forced_streamed_variant(W, G, inc, dtype):
             P = -(-N // M)
             if W * P > 1024:
                 continue
-            with api.forced_hybrid(M, P, W, G, 1):
+            with api.forced_hybrid(M, P, W, G, variant):
                 cfg = run_case(dims, sd, dtype, inc, (0, 0, 0), seed=M * 13 + W)
-            assert cfg == (6, M, P, W, G)
+            assert cfg == (5 + variant, M, P, W, G)
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])

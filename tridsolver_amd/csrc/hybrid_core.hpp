@@ -150,16 +150,22 @@ __device__ __forceinline__ void to_fetch(const Lane &L, TR (&v)[M], TR *tile) {
   __syncthreads();
 }
 
-// Two alternating LDS buffers of `slot` bytes each.
+// LDS exchange buffers of `slot` bytes. With two slots they alternate and an
+// exchange needs one barrier; with one slot (nslot = 1, when LDS is short) the
+// reader calls release() to add the barrier that protects the next write.
 struct Ring {
   unsigned char *base;
   int slot;
   int cur;
+  int nslot = 2;
   template <typename X>
   __device__ __forceinline__ X *next() {
     X *q = reinterpret_cast<X *>(base + (size_t)cur * slot);
-    cur ^= 1;
+    if (nslot == 2) cur ^= 1;
     return q;
+  }
+  __device__ __forceinline__ void release() const {
+    if (nslot == 1) __syncthreads();
   }
 };
 
@@ -222,6 +228,7 @@ __device__ __forceinline__ void reduced_solve(const Lane &L, Ring &ring, TR al_e
     const int sl = t + ps;
     nd = q[sl]; na = q[T + sl]; ng = q[2 * T + sl];
   }
+  ring.release();
   TR A = al_e, Cc = -ga_e * ng, D = de_e - ga_e * nd;
   {
     const TR inv = TR(1) / (TR(1) - ga_e * na);
@@ -240,6 +247,7 @@ __device__ __forceinline__ void reduced_solve(const Lane &L, Ring &ring, TR al_e
       const int sl = t + h * ps;
       Ap = Q[sl]; Cp = Q[T + sl]; Dp = Q[2 * T + sl];
     }
+    ring.release();
     const TR inv = TR(1) / (TR(1) - A * Cm - Cc * Ap);
     const TR nA = -inv * A * Am;
     const TR nC = -inv * Cc * Cp;
@@ -251,6 +259,7 @@ __device__ __forceinline__ void reduced_solve(const Lane &L, Ring &ring, TR al_e
   Q[t] = y;
   __syncthreads();
   ym = (p > 0) ? Q[t - ps] : TR(0);
+  ring.release();
 }
 
 // (iv) x_r = dp_r - ap_r ym - gp_r y; x_{M-1} = y.

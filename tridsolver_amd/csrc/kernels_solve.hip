@@ -222,7 +222,7 @@ static int max_threads_for_m(int M) {
 
 template <typename REAL>
 bool hybrid_cfg_valid(const Geom &g, const HybridCfg &cfg) {
-  if (cfg.V == 1) return stream_cfg_valid<REAL>(g, cfg);
+  if (cfg.V == 1 || cfg.V == 2) return stream_cfg_valid<REAL>(g, cfg);
   if (cfg.V != 0) return false;
   const int maxT = max_threads_for_m<REAL>(cfg.M);
   if (maxT == 0 || cfg.P < 1 || cfg.W < 1) return false;
@@ -274,7 +274,7 @@ template <typename REAL, bool INC>
 hipError_t launch_hybrid(const Geom &g, const HybridCfg &cfg, const REAL *a,
                          const REAL *b, const REAL *c, REAL *d, REAL *u,
                          hipStream_t st) {
-  if (cfg.V == 1) return launch_stream<REAL, INC>(g, cfg, a, b, c, d, u, st);
+  if (cfg.V == 1 || cfg.V == 2) return launch_stream<REAL, INC>(g, cfg, a, b, c, d, u, st);
   const bool contig = g.stride == 1;
 #define TRID_HYB_CASE(MM)                                                        \
   case MM:                                                                       \

@@ -283,7 +283,8 @@ EXTERN_C size_t tridHipWorkspaceBytes(int fp64, int inc, int ndim, int solvedim,
 }
 
 EXTERN_C tridStatus_t tridHipLastConfig(int *algorithm, int *M, int *P, int *W, int *G) {
-  if (algorithm) *algorithm = tl_last_algo == 3 && tl_last_v == 1 ? 6 : tl_last_algo;
+  if (algorithm)
+    *algorithm = tl_last_algo == 3 && tl_last_v > 0 ? 5 + tl_last_v : tl_last_algo;
   if (M) *M = tl_last_m;
   if (P) *P = tl_last_p;
   if (W) *W = tl_last_w;
@@ -296,7 +297,7 @@ EXTERN_C tridStatus_t tridHipForceHybridConfig(int M, int P, int W, int G, int v
     tl_forced = HybridCfg();
     return TRID_STATUS_SUCCESS;
   }
-  if (M < 0 || P < 1 || W < 1 || G < 1 || variant < 0 || variant > 1)
+  if (M < 0 || P < 1 || W < 1 || G < 1 || variant < 0 || variant > 2)
     return TRID_STATUS_INVALID_VALUE;
   tl_forced.M = M;
   tl_forced.P = P;

@@ -162,9 +162,11 @@ std::vector<HybridCfg> candidates(const Geom &g) {
         if ((int64_t)W > g.n0 && W > 1) continue;
         if ((int64_t)c.W * c.P < 32 && c.W < g.n0) continue;  // under-filled workgroups
         if (hybrid_cfg_valid<REAL>(g, c)) out.push_back(c);
-        if (g.stride != 1) {  // streamed strided variant of the same shape
-          c.V = 1;
-          if (hybrid_cfg_valid<REAL>(g, c)) out.push_back(c);
+        if (g.stride != 1) {  // streamed strided variants of the same shape
+          for (int V : {1, 2}) {
+            c.V = V;
+            if (hybrid_cfg_valid<REAL>(g, c)) out.push_back(c);
+          }
         }
       }
   if (g.stride != 1) {  // the streamed variant also runs M = 32 (FP32)
@@ -175,10 +177,12 @@ std::vector<HybridCfg> candidates(const Geom &g) {
         c.P = (int)((g.N + 31) / 32);
         c.W = W;
         c.G = G;
-        c.V = 1;
         if ((int64_t)W * (int64_t)sizeof(REAL) < 64 && W < g.n0) continue;
         if ((int64_t)W > g.n0 && W > 1) continue;
-        if (hybrid_cfg_valid<REAL>(g, c)) out.push_back(c);
+        for (int V : {1, 2}) {
+          c.V = V;
+          if (hybrid_cfg_valid<REAL>(g, c)) out.push_back(c);
+        }
       }
   }
   return out;
