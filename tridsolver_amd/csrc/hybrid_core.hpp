@@ -125,6 +125,100 @@ __device__ __forceinline__ void store_add(const Lane &L, TO *__restrict__ dst,
   }
 }
 
+// ---- 16-byte vector access for contiguous systems --------------------------
+// Fetch order with vectors of V elements: slot q*V + k of lane p holds element
+// V*(p + P*q) + k. Used when N and the row pitch are multiples of V and every
+// array is 16-byte aligned, so a whole vector is either inside [0, N) or
+// beyond it.
+template <typename TR, int V>
+struct VecOf;
+template <>
+struct VecOf<double, 2> {
+  typedef double type __attribute__((ext_vector_type(2)));
+};
+template <>
+struct VecOf<float, 4> {
+  typedef float type __attribute__((ext_vector_type(4)));
+};
+
+template <int M, int V>
+__device__ __forceinline__ int64_t vfetch_row(const Lane &L, int q) {
+  return (int64_t)V * ((int64_t)L.p + (int64_t)L.P * q);
+}
+
+template <int M, int V, typename TR>
+__device__ __forceinline__ void vfetch(const Lane &L, const TR *__restrict__ src, TR (&v)[M],
+                                       TR fill, int64_t lo, int64_t hi) {
+  typedef typename VecOf<TR, V>::type vec;
+#pragma unroll
+  for (int q = 0; q < M / V; ++q) {
+    const int64_t e0 = vfetch_row<M, V>(L, q);
+    if (L.active && e0 < L.N) {
+      const vec x = *reinterpret_cast<const vec *>(src + L.base + e0);
+#pragma unroll
+      for (int k = 0; k < V; ++k) {
+        const int64_t e = e0 + k;
+        v[q * V + k] = (e >= lo && e < hi) ? x[k] : fill;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < V; ++k) v[q * V + k] = fill;
+    }
+  }
+}
+
+template <int M, int V, typename TR>
+__device__ __forceinline__ void vfetch_abcd(const Lane &L, const TR *a, const TR *b,
+                                            const TR *c, const TR *d, TR (&ra)[M],
+                                            TR (&rb)[M], TR (&rc)[M], TR (&rd)[M]) {
+  vfetch<M, V>(L, a, ra, TR(0), 1, L.N);
+  vfetch<M, V>(L, b, rb, TR(1), 0, L.N);
+  vfetch<M, V>(L, c, rc, TR(0), 0, L.N - 1);
+  vfetch<M, V>(L, d, rd, TR(0), 0, L.N);
+}
+
+template <int M, int V, bool ADD, typename TR>
+__device__ __forceinline__ void vstore(const Lane &L, TR *__restrict__ dst, const TR (&v)[M]) {
+  typedef typename VecOf<TR, V>::type vec;
+  if (!L.active) return;
+#pragma unroll
+  for (int q = 0; q < M / V; ++q) {
+    const int64_t e0 = vfetch_row<M, V>(L, q);
+    if (e0 < L.N) {
+      vec *p = reinterpret_cast<vec *>(dst + L.base + e0);
+      vec x;
+      if (ADD) x = *p;
+#pragma unroll
+      for (int k = 0; k < V; ++k) x[k] = ADD ? x[k] + v[q * V + k] : v[q * V + k];
+      *p = x;
+    }
+  }
+}
+
+template <int M, int V, typename TR>
+__device__ __forceinline__ void vto_chunk(const Lane &L, TR (&v)[M], TR *tile) {
+#pragma unroll
+  for (int q = 0; q < M / V; ++q)
+#pragma unroll
+    for (int k = 0; k < V; ++k) tile[skew<M>((int)vfetch_row<M, V>(L, q) + k)] = v[q * V + k];
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < M; ++r) v[r] = tile[L.p * (M + 1) + r];
+  __syncthreads();
+}
+
+template <int M, int V, typename TR>
+__device__ __forceinline__ void vto_fetch(const Lane &L, TR (&v)[M], TR *tile) {
+#pragma unroll
+  for (int r = 0; r < M; ++r) tile[L.p * (M + 1) + r] = v[r];
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < M / V; ++q)
+#pragma unroll
+    for (int k = 0; k < V; ++k) v[q * V + k] = tile[skew<M>((int)vfetch_row<M, V>(L, q) + k)];
+  __syncthreads();
+}
+
 // Fetch order -> chunk order (contiguous systems). `tile` is this system's
 // P*(M+1)-element LDS tile.
 template <int M, bool CONTIG, typename TR>

@@ -48,6 +48,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cstdint>
 
 #include "hybrid_core.hpp"
 #include "trid_internal.h"
@@ -149,11 +150,12 @@ constexpr int hybrid_max_threads() {
   return HybridBounds<REAL, M>::threads;
 }
 
-template <typename REAL, int M, bool INC, bool CONTIG>
+template <typename REAL, int M, bool INC, bool CONTIG, bool VEC>
 __global__ __launch_bounds__((HybridBounds<REAL, M>::threads)) void hybrid_kernel(
     Geom g, int P, int W, int G, const REAL *__restrict__ a,
     const REAL *__restrict__ b, const REAL *__restrict__ c, REAL *__restrict__ d,
     REAL *__restrict__ u) {
+  constexpr int V = 16 / sizeof(REAL);
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const hyb::Lane L = hyb::lane_of<CONTIG>(g, P, W, G);
   const size_t tile_bytes = CONTIG ? sizeof(REAL) * (size_t)W * P * (M + 1) : 0;
@@ -161,26 +163,39 @@ __global__ __launch_bounds__((HybridBounds<REAL, M>::threads)) void hybrid_kerne
   hyb::Ring ring{smem_raw + tile_bytes, (int)(3 * sizeof(REAL) * L.T), 0};
 
   REAL ra[M], rb[M], rc[M], rd[M];
-  hyb::fetch_abcd<M, CONTIG>(L, a, b, c, d, ra, rb, rc, rd);
-  hyb::to_chunk<M, CONTIG>(L, ra, tile);
-  hyb::to_chunk<M, CONTIG>(L, rb, tile);
-  hyb::to_chunk<M, CONTIG>(L, rc, tile);
-  hyb::to_chunk<M, CONTIG>(L, rd, tile);
+  if constexpr (VEC) {
+    hyb::vfetch_abcd<M, V>(L, a, b, c, d, ra, rb, rc, rd);
+    hyb::vto_chunk<M, V>(L, ra, tile);
+    hyb::vto_chunk<M, V>(L, rb, tile);
+    hyb::vto_chunk<M, V>(L, rc, tile);
+    hyb::vto_chunk<M, V>(L, rd, tile);
+  } else {
+    hyb::fetch_abcd<M, CONTIG>(L, a, b, c, d, ra, rb, rc, rd);
+    hyb::to_chunk<M, CONTIG>(L, ra, tile);
+    hyb::to_chunk<M, CONTIG>(L, rb, tile);
+    hyb::to_chunk<M, CONTIG>(L, rc, tile);
+    hyb::to_chunk<M, CONTIG>(L, rd, tile);
+  }
   REAL x[M];
   hyb::solve_chunk<REAL, M>(L, ring, ra, rb, rc, rd, x);
-  hyb::to_fetch<M, CONTIG>(L, x, tile);
-  if (INC) hyb::store_add<M, CONTIG>(L, u, x);
-  else hyb::store<M, CONTIG>(L, d, x);
+  if constexpr (VEC) {
+    hyb::vto_fetch<M, V>(L, x, tile);
+    hyb::vstore<M, V, INC>(L, INC ? u : d, x);
+  } else {
+    hyb::to_fetch<M, CONTIG>(L, x, tile);
+    if (INC) hyb::store_add<M, CONTIG>(L, u, x);
+    else hyb::store<M, CONTIG>(L, d, x);
+  }
 }
 
-template <typename REAL, int M, bool INC, bool CONTIG>
+template <typename REAL, int M, bool INC, bool CONTIG, bool VEC>
 hipError_t launch_hybrid_m(const Geom &g, const HybridCfg &cfg, const REAL *a,
                            const REAL *b, const REAL *c, REAL *d, REAL *u,
                            hipStream_t st) {
   const int T = cfg.W * cfg.P;
   size_t lds = sizeof(REAL) * (size_t)6 * T;
   if (CONTIG) lds += sizeof(REAL) * (size_t)cfg.W * cfg.P * (M + 1);
-  auto kern = hybrid_kernel<REAL, M, INC, CONTIG>;
+  auto kern = hybrid_kernel<REAL, M, INC, CONTIG, VEC>;
   static std::atomic<size_t> lds_granted{0};
   {
     hipError_t e = ensure_lds(reinterpret_cast<const void *>(kern), lds, lds_granted);
@@ -276,10 +291,21 @@ hipError_t launch_hybrid(const Geom &g, const HybridCfg &cfg, const REAL *a,
                          hipStream_t st) {
   if (cfg.V == 1 || cfg.V == 2) return launch_stream<REAL, INC>(g, cfg, a, b, c, d, u, st);
   const bool contig = g.stride == 1;
-#define TRID_HYB_CASE(MM)                                                        \
-  case MM:                                                                       \
-    return contig ? launch_hybrid_m<REAL, MM, INC, true>(g, cfg, a, b, c, d, u, st) \
-                  : launch_hybrid_m<REAL, MM, INC, false>(g, cfg, a, b, c, d, u, st);
+  // 16-byte vector access on contiguous systems when every vector lies wholly
+  // inside or wholly beyond a row: N and the row pitches multiples of V,
+  // 16-byte aligned arrays, M a multiple of V.
+  constexpr int V = 16 / sizeof(REAL);
+  auto al16 = [](const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+  const bool vec = contig && cfg.M % V == 0 && g.N % V == 0 && g.st0 % V == 0 &&
+                   g.st1 % V == 0 && al16(a) && al16(b) && al16(c) && al16(d) &&
+                   (!INC || al16(u));
+#define TRID_HYB_CASE(MM)                                                               \
+  case MM:                                                                              \
+    if (!contig) return launch_hybrid_m<REAL, MM, INC, false, false>(g, cfg, a, b, c, d, u, st); \
+    if constexpr (MM % V == 0) {                                                        \
+      if (vec) return launch_hybrid_m<REAL, MM, INC, true, true>(g, cfg, a, b, c, d, u, st);    \
+    }                                                                                   \
+    return launch_hybrid_m<REAL, MM, INC, true, false>(g, cfg, a, b, c, d, u, st);
   switch (cfg.M) {
     TRID_HYB_CASE(1)
     TRID_HYB_CASE(2)
