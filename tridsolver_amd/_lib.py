@@ -87,6 +87,10 @@ SIGNATURES = {
     "tridAdiStepS": ([P, F] + [P] * 5 + [I, I, I], I),
     "tridAdiRunD": ([P, D] + [P] * 5 + [I, I, I, I, I], I),
     "tridAdiRunS": ([P, F] + [P] * 5 + [I, I, I, I, I], I),
+    "tridAdiStepFusedD": ([D, P, P, I, I, I], I),
+    "tridAdiStepFusedS": ([F, P, P, I, I, I], I),
+    "tridAdiRunFusedD": ([P, D, P, P, I, I, I, I, I], I),
+    "tridAdiRunFusedS": ([P, F, P, P, I, I, I, I, I], I),
 }
 
 _lib = None

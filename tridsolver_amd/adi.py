@@ -60,8 +60,8 @@ def initial_u(nx, ny, nz, dtype, kind="random", seed=0, device="cuda"):
     else:
         g = torch.Generator(device=device)
         g.manual_seed(seed)
-        u[1:-1, 1:-1, 1:-1] = torch.rand(nz - 2, ny - 2, nx - 2, dtype=dtype, device=device,
-                                         generator=g)
+        inner = [max(0, n - 2) for n in (nz, ny, nx)]
+        u[1:-1, 1:-1, 1:-1] = torch.rand(*inner, dtype=dtype, device=device, generator=g)
     return u.reshape(-1).contiguous()
 
 
@@ -92,6 +92,17 @@ def run(lam, u, ws, iters, graph=True, opts=(0, 0, 0), sync=False):
     nx, ny, nz = ws.shape
     check(fn(C.byref(p), lam, _p(u), _p(ws.du), _p(ws.a), _p(ws.b), _p(ws.c), nx, ny, nz,
              iters, 1 if graph else 0), "tridAdiRun")
+
+
+def run_fused(lam, u, du, iters, shape, graph=True, sync=False):
+    """`iters` matrix-free steps (tridAdiRunFused*): du is workspace only."""
+    sfx = "D" if u.dtype == torch.float64 else "S"
+    fn = getattr(_lib.lib(), "tridAdiRunFused" + sfx)
+    use_torch_stream()
+    p = make_params((0, 0, 0), 1 if sync else 0)
+    nx, ny, nz = shape
+    check(fn(C.byref(p), lam, _p(u), _p(du), nx, ny, nz, iters, 1 if graph else 0),
+          "tridAdiRunFused")
 
 
 def _phase_times(lam, u, ws, opts, iters):

@@ -47,7 +47,8 @@ struct Lane {
   int64_t base;  // element offset of the system's first row
   int64_t N;     // system length
   int64_t stride;
-  bool active;   // the lane belongs to an existing system
+  int64_t i0, i1;  // the system's coordinates in the n0 x n1 grid of systems
+  bool active;     // the lane belongs to an existing system
 };
 
 // Lane of a batched (Geom) launch: system groups of W x-neighbours (strided)
@@ -70,6 +71,8 @@ __device__ __forceinline__ Lane lane_of(const Geom &g, int P, int W, int G) {
   L.base = i0 * g.st0 + i1 * g.st1;
   L.N = g.N;
   L.stride = g.stride;
+  L.i0 = i0;
+  L.i1 = i1;
   return L;
 }
 

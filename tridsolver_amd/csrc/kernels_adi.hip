@@ -22,7 +22,9 @@
 #include "../../include/tridsolver.h"
 #include "../../include/tridsolver_adi.h"
 #include "../../include/tridsolver_hip.h"
+#include "hybrid_core.hpp"
 #include "trid_internal.h"
+#include "trid_tune.h"
 
 namespace trid {
 namespace {
@@ -64,7 +66,7 @@ __global__ __launch_bounds__(PRE_BX *PRE_BY) void adi_preproc_kernel(
       d = 0; a = 0; b = 1; c = 0;
     }
     du[ind] = d;
-    c0.a[ind] = a; c0.b[ind] = b; c0.c[ind] = c;
+    if (c0.a) { c0.a[ind] = a; c0.b[ind] = b; c0.c[ind] = c; }
     if (c1.a) { c1.a[ind] = a; c1.b[ind] = b; c1.c[ind] = c; }
     if (c2.a) { c2.a[ind] = a; c2.b[ind] = b; c2.c[ind] = c; }
     um = uc;
@@ -76,7 +78,8 @@ template <typename REAL>
 tridStatus_t preproc(REAL lambda, const REAL *u, REAL *du, REAL *ax, REAL *bx, REAL *cx,
                      REAL *ay, REAL *by, REAL *cy, REAL *az, REAL *bz, REAL *cz, int nx,
                      int ny, int nz) {
-  if (!u || !du || !ax || !bx || !cx || nx < 1 || ny < 1 || nz < 1)
+  const bool x_null = !ax && !bx && !cx;  // du only (matrix-free step)
+  if (!u || !du || nx < 1 || ny < 1 || nz < 1 || (!x_null && (!ax || !bx || !cx)))
     return TRID_STATUS_INVALID_VALUE;
   // A direction whose three pointers are NULL, or equal to an earlier set,
   // is not written again.
@@ -87,6 +90,7 @@ tridStatus_t preproc(REAL lambda, const REAL *u, REAL *du, REAL *ax, REAL *bx, R
   const bool y_null = !ay && !by && !cy, z_null = !az && !bz && !cz;
   if ((!y_null && (!ay || !by || !cy)) || (!z_null && (!az || !bz || !cz)))
     return TRID_STATUS_INVALID_VALUE;
+  if (x_null && (!y_null || !z_null)) return TRID_STATUS_INVALID_VALUE;
   if (!y_null && !same(ay, by, cy, ax, bx, cx)) c1 = {ay, by, cy};
   if (!z_null && !same(az, bz, cz, ax, bx, cx) && !same(az, bz, cz, ay, by, cy))
     c2 = {az, bz, cz};
@@ -99,6 +103,153 @@ tridStatus_t preproc(REAL lambda, const REAL *u, REAL *du, REAL *ax, REAL *bx, R
                      current_stream(), lambda, u, du, c0, c1, c2, nx, ny, nz);
   return hipGetLastError() == hipSuccess ? TRID_STATUS_SUCCESS
                                          : TRID_STATUS_EXECUTION_FAILED;
+}
+
+// ---------------------------------------------------------------------------
+// Matrix-free sweeps for the ADI step. The coefficients preproc would store
+// are functions of the grid position only (interior: a = c = -0.5 lambda,
+// b = 1 + lambda; any boundary point: a = 0, b = 1, c = 0), so each sweep
+// computes them from (i, j, k) with the same expressions in the same
+// precision, and reads only du (and u for the z sweep): 16 B per point
+// per sweep instead of 40 (FP64). The solve is the Thomas-PCR hybrid of
+// hybrid_core.hpp, so results equal the array-based step bit for bit.
+// ---------------------------------------------------------------------------
+template <typename REAL>
+struct AdiCoef {
+  REAL lambda;
+  int nx, ny, nz, sd;  // sd: direction of the sweep
+};
+
+// Coefficients of chunk-order rows r = 0..M-1 of the lane's system.
+template <typename REAL, int M, bool CONTIG>
+__device__ __forceinline__ void adi_coefs(const hyb::Lane &L, const AdiCoef<REAL> &k,
+                                          REAL (&ra)[M], REAL (&rb)[M], REAL (&rc)[M]) {
+  const REAL ca = REAL(-0.5) * k.lambda, cb = REAL(1) + k.lambda;
+  // coordinates of the system; the row index runs along direction sd
+  int ci, cj, ck;
+  if (k.sd == 0) { cj = (int)L.i0; ck = (int)L.i1; ci = 0; }
+  else if (k.sd == 1) { ci = (int)L.i0; ck = (int)L.i1; cj = 0; }
+  else { ci = (int)L.i0; cj = (int)L.i1; ck = 0; }
+#pragma unroll
+  for (int r = 0; r < M; ++r) {
+    const int64_t row = (int64_t)L.p * M + r;
+    int i = ci, j = cj, kk = ck;
+    if (k.sd == 0) i = (int)row;
+    else if (k.sd == 1) j = (int)row;
+    else kk = (int)row;
+    const bool pad = !L.active || row >= L.N;
+    const bool edge = i == 0 || i == k.nx - 1 || j == 0 || j == k.ny - 1 || kk == 0 ||
+                      kk == k.nz - 1;
+    const bool inner = !pad && !edge;
+    ra[r] = inner ? ca : REAL(0);
+    rb[r] = inner ? cb : REAL(1);
+    rc[r] = inner ? ca : REAL(0);
+  }
+}
+
+template <typename REAL, int M>
+struct AdiBounds {  // same register budget as the array-based hybrid kernel
+  static constexpr int threads = sizeof(REAL) * M > 64 ? 512 : 1024;
+};
+
+template <typename REAL, int M, bool INC, bool CONTIG>
+__global__ __launch_bounds__((AdiBounds<REAL, M>::threads)) void adi_solve_kernel(Geom g, int P, int W, int G,
+                                                         AdiCoef<REAL> kc, REAL *__restrict__ d,
+                                                         REAL *__restrict__ u) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const hyb::Lane L = hyb::lane_of<CONTIG>(g, P, W, G);
+  const size_t tile_bytes = CONTIG ? sizeof(REAL) * (size_t)W * P * (M + 1) : 0;
+  REAL *tile = reinterpret_cast<REAL *>(smem_raw) + (size_t)L.w * P * (M + 1);
+  hyb::Ring ring{smem_raw + tile_bytes, (int)(3 * sizeof(REAL) * L.T), 0};
+  REAL ra[M], rb[M], rc[M], rd[M];
+  hyb::fetch<M, CONTIG>(L, d, rd, REAL(0), 0, 0, L.N);
+  hyb::to_chunk<M, CONTIG>(L, rd, tile);
+  adi_coefs<REAL, M, CONTIG>(L, kc, ra, rb, rc);
+  REAL x[M];
+  hyb::solve_chunk<REAL, M>(L, ring, ra, rb, rc, rd, x);
+  hyb::to_fetch<M, CONTIG>(L, x, tile);
+  if (INC) hyb::store_add<M, CONTIG>(L, u, x);
+  else hyb::store<M, CONTIG>(L, d, x);
+}
+
+template <typename REAL, int M, bool INC, bool CONTIG>
+hipError_t launch_adi_solve_m(const Geom &g, const HybridCfg &cfg, const AdiCoef<REAL> &kc,
+                              REAL *d, REAL *u, hipStream_t st) {
+  const int T = cfg.W * cfg.P;
+  size_t lds = hyb::ring_bytes<REAL>(T, 3);
+  if (CONTIG) lds += sizeof(REAL) * (size_t)T * (M + 1);
+  auto kern = adi_solve_kernel<REAL, M, INC, CONTIG>;
+  static std::atomic<size_t> lds_granted{0};
+  {
+    hipError_t e = ensure_lds(reinterpret_cast<const void *>(kern), lds, lds_granted);
+    if (e != hipSuccess) return e;
+  }
+  const int64_t blocks = (g.n0 + cfg.W - 1) / cfg.W * g.n1;
+  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(T), lds, st, g, cfg.P, cfg.W, cfg.G,
+                     kc, d, u);
+  return hipGetLastError();
+}
+
+// Shape: the one the array-based solve of this geometry uses (tuned or
+// default), with plain loads. Systems longer than one workgroup covers are
+// not supported here.
+template <typename REAL, bool INC>
+bool adi_cfg(int dev, const Geom &g, HybridCfg *cfg) {
+  if (!tuned_config<REAL, INC>(dev, g, cfg)) return false;
+  cfg->V = 0;
+  if (hybrid_cfg_valid<REAL>(g, *cfg)) return true;
+  return hybrid_default_cfg<REAL>(g, cfg) && cfg->V == 0;
+}
+
+template <typename REAL, bool INC>
+tridStatus_t adi_sweep(int dev, REAL lambda, REAL *d, REAL *u, int nx, int ny, int nz,
+                       int sd) {
+  const int dims[3] = {nx, ny, nz};
+  Geom g;
+  bool empty = false;
+  tridStatus_t s = make_geom(3, sd, dims, dims, &g, &empty);
+  if (s != TRID_STATUS_SUCCESS || empty) return s;
+  HybridCfg cfg;
+  if (!adi_cfg<REAL, INC>(dev, g, &cfg)) return TRID_STATUS_INVALID_VALUE;
+  AdiCoef<REAL> kc{lambda, nx, ny, nz, sd};
+  hipStream_t st = current_stream();
+  const bool contig = sd == 0;
+  hipError_t e;
+#define TRID_ADI_CASE(MM)                                                              \
+  case MM:                                                                             \
+    e = contig ? launch_adi_solve_m<REAL, MM, INC, true>(g, cfg, kc, d, u, st)         \
+               : launch_adi_solve_m<REAL, MM, INC, false>(g, cfg, kc, d, u, st);        \
+    break;
+  switch (cfg.M) {
+    TRID_ADI_CASE(1)
+    TRID_ADI_CASE(2)
+    TRID_ADI_CASE(4)
+    TRID_ADI_CASE(8)
+    TRID_ADI_CASE(16)
+    default:
+      return TRID_STATUS_INVALID_VALUE;
+  }
+#undef TRID_ADI_CASE
+  if (e != hipSuccess) {
+    record_hip_error(e);
+    return TRID_STATUS_EXECUTION_FAILED;
+  }
+  return TRID_STATUS_SUCCESS;
+}
+
+// Matrix-free step: du = preproc(u); du = solve_x(du); du = solve_y(du);
+// u += solve_z(du).
+template <typename REAL>
+tridStatus_t step_fused(REAL lambda, REAL *u, REAL *du, int nx, int ny, int nz) {
+  int dev = 0;
+  tridStatus_t s = check_device(&dev);
+  if (s == TRID_STATUS_SUCCESS)
+    s = preproc<REAL>(lambda, u, du, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                      nullptr, nullptr, nullptr, nx, ny, nz);
+  if (s == TRID_STATUS_SUCCESS) s = adi_sweep<REAL, false>(dev, lambda, du, u, nx, ny, nz, 0);
+  if (s == TRID_STATUS_SUCCESS) s = adi_sweep<REAL, false>(dev, lambda, du, u, nx, ny, nz, 1);
+  if (s == TRID_STATUS_SUCCESS) s = adi_sweep<REAL, true>(dev, lambda, du, u, nx, ny, nz, 2);
+  return s;
 }
 
 // One step, stream-ordered, no host synchronisation (ctx->sync forced to 0
@@ -140,13 +291,17 @@ tridStatus_t step<float>(const TridParams *ctx, float lambda, float *u, float *d
 template <typename REAL>
 tridStatus_t run(const TridParams *ctx, REAL lambda, REAL *u, REAL *du, REAL *a, REAL *b,
                  REAL *c, int nx, int ny, int nz, int iters, int use_graph) {
+  // a == NULL selects the matrix-free step
+  auto one_step = [&]() {
+    return a ? step<REAL>(ctx, lambda, u, du, a, b, c, nx, ny, nz)
+             : step_fused<REAL>(lambda, u, du, nx, ny, nz);
+  };
   if (iters < 0) return TRID_STATUS_INVALID_VALUE;
   hipStream_t st = current_stream();
   tridStatus_t s = TRID_STATUS_SUCCESS;
   const bool sync = ctx == nullptr || ctx->sync != 0;
   if (!use_graph || iters == 0) {
-    for (int it = 0; it < iters && s == TRID_STATUS_SUCCESS; ++it)
-      s = step<REAL>(ctx, lambda, u, du, a, b, c, nx, ny, nz);
+    for (int it = 0; it < iters && s == TRID_STATUS_SUCCESS; ++it) s = one_step();
   } else {
     // Capture one step on a private stream (the null stream cannot be
     // captured), replay it `iters` times on the caller's stream.
@@ -162,7 +317,7 @@ tridStatus_t run(const TridParams *ctx, REAL lambda, REAL *u, REAL *du, REAL *a,
         s = TRID_STATUS_EXECUTION_FAILED;
       } else {
         tridHipSetStream(cap);
-        tridStatus_t s_step = step<REAL>(ctx, lambda, u, du, a, b, c, nx, ny, nz);
+        tridStatus_t s_step = one_step();
         tridHipSetStream(prev);
         hipError_t e = hipStreamEndCapture(cap, &graph);
         if (s_step != TRID_STATUS_SUCCESS) s = s_step;
@@ -226,14 +381,43 @@ EXTERN_C tridStatus_t tridAdiStepS(const TridParams *ctx, float lambda, float *u
   return s;
 }
 
+EXTERN_C tridStatus_t tridAdiStepFusedD(double lambda, double *u, double *du, int nx, int ny,
+                                        int nz) {
+  if (!u || !du) return TRID_STATUS_INVALID_VALUE;
+  return step_fused<double>(lambda, u, du, nx, ny, nz);
+}
+
+EXTERN_C tridStatus_t tridAdiStepFusedS(float lambda, float *u, float *du, int nx, int ny,
+                                        int nz) {
+  if (!u || !du) return TRID_STATUS_INVALID_VALUE;
+  return step_fused<float>(lambda, u, du, nx, ny, nz);
+}
+
+EXTERN_C tridStatus_t tridAdiRunFusedD(const TridParams *ctx, double lambda, double *u,
+                                       double *du, int nx, int ny, int nz, int iters,
+                                       int use_graph) {
+  if (!u || !du) return TRID_STATUS_INVALID_VALUE;
+  return run<double>(ctx, lambda, u, du, nullptr, nullptr, nullptr, nx, ny, nz, iters,
+                     use_graph);
+}
+
+EXTERN_C tridStatus_t tridAdiRunFusedS(const TridParams *ctx, float lambda, float *u, float *du,
+                                       int nx, int ny, int nz, int iters, int use_graph) {
+  if (!u || !du) return TRID_STATUS_INVALID_VALUE;
+  return run<float>(ctx, lambda, u, du, nullptr, nullptr, nullptr, nx, ny, nz, iters,
+                    use_graph);
+}
+
 EXTERN_C tridStatus_t tridAdiRunD(const TridParams *ctx, double lambda, double *u,
                                   double *du, double *a, double *b, double *c, int nx,
                                   int ny, int nz, int iters, int use_graph) {
+  if (!a || !b || !c) return TRID_STATUS_INVALID_VALUE;
   return run<double>(ctx, lambda, u, du, a, b, c, nx, ny, nz, iters, use_graph);
 }
 
 EXTERN_C tridStatus_t tridAdiRunS(const TridParams *ctx, float lambda, float *u, float *du,
                                   float *a, float *b, float *c, int nx, int ny, int nz,
                                   int iters, int use_graph) {
+  if (!a || !b || !c) return TRID_STATUS_INVALID_VALUE;
   return run<float>(ctx, lambda, u, du, a, b, c, nx, ny, nz, iters, use_graph);
 }
