@@ -45,7 +45,8 @@ EXTERN_C size_t tridHipWorkspaceBytes(int fp64, int inc, int ndim, int solvedim,
 
 /* What the last solve of this thread ran: algorithm (1 per-lane Thomas,
  * 2 mixed precision, 3 hybrid, 4 adjoint, 5 ragged, 6 streamed strided
- * hybrid, 7 streamed strided hybrid with LDS storage), and for the hybrid its
+ * hybrid, 7 streamed strided hybrid with LDS storage, 8 wide strided hybrid),
+ * and for the hybrid its
  * rows per lane M, lanes per system P, systems per workgroup W and XCD
  * grouping G (consecutive system groups dealt to one XCD; 1 = plain order). */
 EXTERN_C tridStatus_t tridHipLastConfig(int *algorithm, int *M, int *P, int *W, int *G);
@@ -53,7 +54,10 @@ EXTERN_C tridStatus_t tridHipLastConfig(int *algorithm, int *M, int *P, int *W, 
 /* Forces the hybrid shape for this thread's subsequent solves (M = 0 clears).
  * variant 0: plain hybrid; 1: streamed strided variant (strided systems,
  * FP64 M = 8/16, FP32 M = 16/32, up to 1024 threads); 2: as 1 with one
- * sweep value per row kept in LDS (arrays spanning < 4 GiB). A shape that does not
+ * sweep value per row kept in LDS (arrays spanning < 4 GiB); 3: strided
+ * variant with a staged in LDS by global->LDS loads (x-neighbouring strided
+ * systems, W * sizeof = 128 or 256 bytes, M = 16/32, W * P a multiple of 64,
+ * arrays spanning < 4 GiB). A shape that does not
  * fit a call (P*M < length, too many threads or LDS) makes that call return
  * TRID_STATUS_INVALID_VALUE. */
 EXTERN_C tridStatus_t tridHipForceHybridConfig(int M, int P, int W, int G, int variant);

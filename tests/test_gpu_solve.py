@@ -154,6 +154,84 @@ def test_forced_streamed_variant(W, G, inc, variant, dtype):
             assert cfg == (5 + variant, M, P, W, G)
 
 
+def _wide_shape(dtype, M, N, seg=256):
+    """P for the wide variant (whole waves: W*P a multiple of 64), or None if
+    the shape does not fit (threads, LDS)."""
+    esize = np.dtype(dtype).itemsize
+    W = seg // esize
+    per_wave = max(1, 64 // W)
+    P = -(-N // M)
+    P = -(-P // per_wave) * per_wave
+    threads = 512 if dtype == np.float64 else 1024
+    slots = 2 if seg == 256 else 1
+    lds = P * M * seg + slots * 3 * W * P * esize
+    if W * P > threads or lds > 160 * 1024:
+        return None, W
+    return P, W
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("seg", [128, 256])
+@pytest.mark.parametrize("G", [1, 8])
+@pytest.mark.parametrize("inc", [False, True])
+def test_forced_wide_variant(G, inc, seg, dtype):
+    """The strided variant with a staged in LDS by global->LDS loads (128- or
+    256-byte rows): both compiled M, y and z, partial workgroups (n0 not a
+    multiple of W, n0 < W), lengths not multiples of M, N up to 512, Inc."""
+    cases = (([40, 37, 9], 1), ([17, 4, 51], 2), ([64, 3, 100], 2), ([33, 8, 130], 2),
+             ([70, 256, 2], 1), ([32, 2, 512], 2), ([96, 300, 1], 1), ([20, 3, 61], 2))
+    ran = 0
+    for M in (16, 32):
+        for dims, sd in cases:
+            P, W = _wide_shape(dtype, M, dims[sd], seg)
+            if P is None:
+                continue
+            with api.forced_hybrid(M, P, W, G, 3):
+                cfg = run_case(dims, sd, dtype, inc, (0, 0, 0), seed=M * 7 + dims[sd])
+            assert cfg == (8, M, P, W, G)
+            ran += 1
+    assert ran >= 8
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_forced_wide_padded(dtype):
+    """Wide variant on the interior of a halo-padded allocation."""
+    rng = np.random.default_rng(23)
+    dims = [70, 22, 40]
+    n = int(np.prod(dims))
+    a, b, c, d = H.diag_dominant(rng, n, dtype)
+    arrs = [H.pad_halo(t, dims) for t in (a, b, c, d)]
+    off, pads = H.halo_offset(dims), [t + 2 for t in dims]
+    for sd, seg in ((1, 256), (2, 256), (1, 128), (2, 128)):
+        P, W = _wide_shape(dtype, 16, dims[sd], seg)
+        with api.forced_hybrid(16, P, W, 8, 3):
+            gd, _ = gpu_solve(*arrs, solvedim=sd, dims=dims, pads=pads, offset=off)
+            assert api.last_config()[0] == 8
+        od = arrs[3].copy()
+        oracle.batch(arrs[0], arrs[1], arrs[2], od, ndim=3, solvedim=sd, dims=dims,
+                     pads=pads, offset=off)
+        H.assert_norm_close(gd, od, dtype)
+        H.ref_allclose(od, gd, dtype)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_wide_rejects_unfit_shapes(dtype):
+    """Shapes the wide variant cannot run are refused before any launch:
+    contiguous (x) systems, rows of other than 128 or 256 bytes, partial
+    waves, too few rows."""
+    from tridsolver_amd._lib import TridError
+    rng = np.random.default_rng(5)
+    dims = [64, 32, 32]
+    a, b, c, d = H.diag_dominant(rng, int(np.prod(dims)), dtype)
+    W = 32 if dtype == np.float64 else 64
+    for sd, cfg in ((0, (16, 4, W, 1, 3)), (1, (16, 8, W // 4, 1, 3)),
+                    (2, (16, 1, W, 1, 3))):
+        with api.forced_hybrid(*cfg):
+            with pytest.raises(TridError) as ei:
+                gpu_solve(a, b, c, d, solvedim=sd, dims=dims)
+        assert ei.value.status == 3
+
+
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_edge_lengths(dtype):
     for dims in ([1], [2], [1, 9], [9, 1], [2, 3, 1], [3000], [2500, 2]):

@@ -238,6 +238,7 @@ static int max_threads_for_m(int M) {
 template <typename REAL>
 bool hybrid_cfg_valid(const Geom &g, const HybridCfg &cfg) {
   if (cfg.V == 1 || cfg.V == 2) return stream_cfg_valid<REAL>(g, cfg);
+  if (cfg.V == 3) return wide_cfg_valid<REAL>(g, cfg);
   if (cfg.V != 0) return false;
   const int maxT = max_threads_for_m<REAL>(cfg.M);
   if (maxT == 0 || cfg.P < 1 || cfg.W < 1) return false;
@@ -290,6 +291,7 @@ hipError_t launch_hybrid(const Geom &g, const HybridCfg &cfg, const REAL *a,
                          const REAL *b, const REAL *c, REAL *d, REAL *u,
                          hipStream_t st) {
   if (cfg.V == 1 || cfg.V == 2) return launch_stream<REAL, INC>(g, cfg, a, b, c, d, u, st);
+  if (cfg.V == 3) return launch_wide<REAL, INC>(g, cfg, a, b, c, d, u, st);
   const bool contig = g.stride == 1;
   // 16-byte vector access on contiguous systems when every vector lies wholly
   // inside or wholly beyond a row: N and the row pitches multiples of V,

@@ -297,7 +297,7 @@ EXTERN_C tridStatus_t tridHipForceHybridConfig(int M, int P, int W, int G, int v
     tl_forced = HybridCfg();
     return TRID_STATUS_SUCCESS;
   }
-  if (M < 0 || P < 1 || W < 1 || G < 1 || variant < 0 || variant > 2)
+  if (M < 0 || P < 1 || W < 1 || G < 1 || variant < 0 || variant > 3)
     return TRID_STATUS_INVALID_VALUE;
   tl_forced.M = M;
   tl_forced.P = P;

@@ -72,6 +72,13 @@ template <typename REAL, bool INC>
 hipError_t launch_stream(const Geom &g, const HybridCfg &cfg, const REAL *a, const REAL *b,
                          const REAL *c, REAL *d, REAL *u, hipStream_t st);
 
+// ---- wide strided variant (kernels_wide.hip) ------------------------------
+template <typename REAL>
+bool wide_cfg_valid(const Geom &g, const HybridCfg &c);
+template <typename REAL, bool INC>
+hipError_t launch_wide(const Geom &g, const HybridCfg &cfg, const REAL *a, const REAL *b,
+                       const REAL *c, REAL *d, REAL *u, hipStream_t st);
+
 // ---- mixed precision (kernels_mixed.hip), FP64 operands ------------------
 bool mixed_cfg_valid(const Geom &g, const HybridCfg &c);
 bool mixed_default_cfg(const Geom &g, HybridCfg *cfg);

@@ -169,6 +169,21 @@ std::vector<HybridCfg> candidates(const Geom &g) {
           }
         }
       }
+  if (g.stride != 1) {  // wide variant: 128- or 256-byte rows, a in LDS
+    for (int seg : {128, 256})
+      for (int M : {16, 32})
+        for (int G : Gs) {
+          HybridCfg c;
+          c.M = M;
+          c.W = seg / (int)sizeof(REAL);
+          const int per_wave = std::max(1, 64 / c.W);  // whole waves
+          c.P = (int)((g.N + M - 1) / M);
+          c.P = (c.P + per_wave - 1) / per_wave * per_wave;
+          c.G = G;
+          c.V = 3;
+          if (hybrid_cfg_valid<REAL>(g, c)) out.push_back(c);
+        }
+  }
   if (g.stride != 1) {  // the streamed variant also runs M = 32 (FP32)
     for (int W : Ws)
       for (int G : Gs) {
