@@ -196,9 +196,19 @@ hipError_t launch_adi_solve_m(const Geom &g, const HybridCfg &cfg, const AdiCoef
 template <typename REAL, bool INC>
 bool adi_cfg(int dev, const Geom &g, HybridCfg *cfg) {
   if (!tuned_config<REAL, INC>(dev, g, cfg)) return false;
-  cfg->V = 0;
-  if (hybrid_cfg_valid<REAL>(g, *cfg)) return true;
-  return hybrid_default_cfg<REAL>(g, cfg) && cfg->V == 0;
+  if (cfg->V != 0) {
+    // a variant shape: keep its chunking (M, P: same arithmetic) with plain
+    // 128-byte rows, which suit a sweep that reads only d
+    HybridCfg c = *cfg;
+    c.V = 0;
+    c.W = 128 / (int)sizeof(REAL);
+    if (hybrid_cfg_valid<REAL>(g, c)) {
+      *cfg = c;
+      return true;
+    }
+    return hybrid_plain_cfg<REAL>(g, cfg);
+  }
+  return hybrid_cfg_valid<REAL>(g, *cfg) || hybrid_plain_cfg<REAL>(g, cfg);
 }
 
 template <typename REAL, bool INC>

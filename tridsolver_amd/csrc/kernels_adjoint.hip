@@ -150,7 +150,7 @@ bool adjoint_default_cfg(const Geom &g, HybridCfg *cfg) {
   // Same shapes as the forward solve where they fit the adjoint's register
   // budget, else the next valid one.
   HybridCfg c;
-  if (hybrid_default_cfg<REAL>(g, &c)) {
+  if (hybrid_plain_cfg<REAL>(g, &c)) {
     for (int k = 0; k < 4 && !adjoint_cfg_valid<REAL>(g, c); ++k) {
       if (g.stride == 1 && c.W > 1) c.W /= 2;
       else if (c.M < 16) { c.M *= 2; c.P = (int)((g.N + c.M - 1) / c.M); }

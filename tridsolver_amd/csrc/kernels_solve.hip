@@ -43,6 +43,9 @@
 //    owns rows p*M..p*M+M-1; the result goes back through the same tile.
 //    Strided systems are loaded directly: W x-neighbouring systems per
 //    workgroup, W consecutive lanes read W consecutive addresses.
+//    kernels_stream.hip and kernels_wide.hip hold strided variants that keep
+//    fewer values in registers (the latter, with a staged in LDS by
+//    global->LDS loads, is the default for strided systems).
 //
 // Division is IEEE (v_div_* sequence) as in the reference CPU code.
 
@@ -251,7 +254,7 @@ bool hybrid_cfg_valid(const Geom &g, const HybridCfg &cfg) {
 }
 
 template <typename REAL>
-bool hybrid_default_cfg(const Geom &g, HybridCfg *cfg) {
+bool hybrid_plain_cfg(const Geom &g, HybridCfg *cfg) {
   const bool contig = g.stride == 1;
   static const int Ms[] = {1, 2, 4, 8, 16};
   if (contig) {
@@ -269,7 +272,7 @@ bool hybrid_default_cfg(const Geom &g, HybridCfg *cfg) {
   }
   // Strided: 128 bytes of neighbouring systems per row (W = 16 FP64 / 32
   // FP32); the smallest M whose workgroup fits the preferred size (FP64: 512
-  // threads at M = 16, measured best at 512^3), else any valid shape.
+  // threads at M = 16), else any valid shape.
   const int pref = sizeof(REAL) == 8 ? 512 : 1024;
   for (int pass = 0; pass < 2; ++pass) {
     for (int W = 128 / (int)sizeof(REAL); W >= 4; W /= 2) {
@@ -284,6 +287,33 @@ bool hybrid_default_cfg(const Geom &g, HybridCfg *cfg) {
     }
   }
   return false;
+}
+
+template <typename REAL>
+bool hybrid_default_cfg(const Geom &g, HybridCfg *cfg) {
+  if (g.stride != 1) {
+    // Strided systems: the variant with a staged in LDS, shapes measured
+    // at 256^3-512^3. Rows at least 64 KiB apart (z) lie in different pages:
+    // FP64 takes 256-byte rows (M = 32 above 256 rows) in plain workgroup
+    // order, FP32 128-byte rows. Rows close together (y) take 128-byte rows,
+    // XCD-grouped.
+    HybridCfg c;
+    const bool fp64 = sizeof(REAL) == 8;
+    const int64_t row_bytes = g.stride * (int64_t)sizeof(REAL);
+    const bool far = row_bytes >= (64 << 10);
+    c.M = (far && fp64 && g.N > 256) ? 32 : 16;
+    c.W = ((far && fp64) ? 256 : 128) / (int)sizeof(REAL);
+    const int per_wave = std::max(1, 64 / c.W);
+    c.P = (int)((g.N + c.M - 1) / c.M);
+    c.P = (c.P + per_wave - 1) / per_wave * per_wave;
+    c.G = !far ? 8 : (fp64 || row_bytes >= (1 << 20)) ? 1 : 8;
+    c.V = 3;
+    if (hybrid_cfg_valid<REAL>(g, c)) {
+      *cfg = c;
+      return true;
+    }
+  }
+  return hybrid_plain_cfg<REAL>(g, cfg);
 }
 
 template <typename REAL, bool INC>
@@ -330,7 +360,8 @@ hipError_t launch_hybrid(const Geom &g, const HybridCfg &cfg, const REAL *a,
   template hipError_t launch_hybrid<REAL, true>(const Geom &, const HybridCfg &, \
       const REAL *, const REAL *, const REAL *, REAL *, REAL *, hipStream_t);    \
   template bool hybrid_cfg_valid<REAL>(const Geom &, const HybridCfg &);         \
-  template bool hybrid_default_cfg<REAL>(const Geom &, HybridCfg *);
+  template bool hybrid_default_cfg<REAL>(const Geom &, HybridCfg *);               \
+  template bool hybrid_plain_cfg<REAL>(const Geom &, HybridCfg *);
 
 TRID_INST(float)
 TRID_INST(double)

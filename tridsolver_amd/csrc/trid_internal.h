@@ -127,6 +127,9 @@ const HybridCfg &forced_config();
 // are too long for one workgroup.
 template <typename REAL>
 bool hybrid_default_cfg(const Geom &g, HybridCfg *cfg);
+// The default among plain-kernel shapes only (V = 0).
+template <typename REAL>
+bool hybrid_plain_cfg(const Geom &g, HybridCfg *cfg);
 template <typename REAL>
 bool hybrid_cfg_valid(const Geom &g, const HybridCfg &cfg);
 
