@@ -113,25 +113,52 @@ def cpu_baseline(n, slab, reps=3):
         else:
             oracle.batch(a, b, c, x, ndim=3, solvedim=sd, dims=dims)
 
-    per_dir = {}
-    x = d.copy()
-    for sd, name in enumerate("xyz"):
-        run(sd, x)  # warm-up
-        ts = []
-        for _ in range(reps):
-            t0 = time.perf_counter()
-            run(sd, x)
-            ts.append(time.perf_counter() - t0)
-        per_dir[name] = statistics.median(ts)
+    def timed():
+        per_dir = {}
+        x = d.copy()
+        for sd, name in enumerate("xyz"):
+            run(sd, x)  # warm-up
+            ts = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                run(sd, x)
+                ts.append(time.perf_counter() - t0)
+            per_dir[name] = statistics.median(ts)
+        return per_dir
+
+    per_dir = timed()
     total = sum(per_dir.values())
     pts_s = 3 * npts / total
+    single = None
+    if kind == "reference":  # the same sample on one OpenMP thread
+        oracle.ref_set_threads(1)
+        one = timed()
+        oracle.ref_set_threads(threads)
+        single = {"value": 3 * npts / sum(one.values()),
+                  "per_dir_gpts_s": {k: round(npts / v / 1e9, 3) for k, v in one.items()}}
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     desc = ("%s CPU solver, FP64 %dx%dx%d sample (x, y, z solves, median of %d), "
-            "%d OpenMP threads, %s" % ("reference" if kind == "reference" else "oracle",
-                                      n, n, slab, reps, threads,
-                                      oracle.ref_isa() if kind == "reference" else "scalar"))
-    return {"value": pts_s, "unit": "grid pts/s", "cores": threads, "kind": kind,
-            "sample": desc,
-            "per_dir_gpts_s": {k: round(n * n * slab / v / 1e9, 3) for k, v in per_dir.items()}}
+            "%d OpenMP threads, %s, %s" % ("reference" if kind == "reference" else "oracle",
+                                          n, n, slab, reps, threads,
+                                          oracle.ref_isa() if kind == "reference" else "scalar",
+                                          model or "cpu model unknown"))
+    out = {"value": pts_s, "unit": "grid pts/s", "cores": threads, "kind": kind,
+           "sample": desc, "alg_GBs": round(40.0 * pts_s / 1e9, 1),
+           "per_dir_gpts_s": {k: round(npts / v / 1e9, 3) for k, v in per_dir.items()},
+           "nproc": os.cpu_count()}
+    if single is not None:
+        out["single_thread"] = {"value": single["value"], "cores": 1,
+                                "alg_GBs": round(40.0 * single["value"] / 1e9, 2),
+                                "per_dir_gpts_s": single["per_dir_gpts_s"]}
+    return out
 
 
 def bench_fp32_dirs(torch, api, n, tune):

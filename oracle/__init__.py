@@ -240,6 +240,13 @@ def ref():
     return _ref
 
 
+def ref_set_threads(n: int) -> None:
+    """OpenMP threads of the reference library's later parallel regions
+    (omp_set_num_threads of the libgomp instance it links)."""
+    ref()
+    C.CDLL("libgomp.so.1").omp_set_num_threads(int(n))
+
+
 def ref_batch(a, b, c, d, u=None, *, ndim, solvedim, dims, pads=None, inc=False,
               offset=0):
     """Calls the reference's own tridDmtsvStridedBatch[Inc] on host arrays."""

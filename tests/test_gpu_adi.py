@@ -186,3 +186,27 @@ def test_fused_rejects_bad_arguments():
     assert L.tridAdiStepFusedD(1.0, p, p, 0, 2, 4) == 3
     # a partial coefficient set stays an error on the array API
     assert L.tridAdiRunD(None, 1.0, p, p, None, p, p, 2, 2, 2, 1, 0) == 3
+
+
+@pytest.mark.parametrize("fused", [0, 1])
+@pytest.mark.parametrize("prec", [64, 32])
+def test_cli_dump_matches_oracle(tmp_path, fused, prec):
+    """The ADI command line with the reference's start (1 on the boundary),
+    its raw dump read back and compared with the oracle ADI."""
+    from tridsolver_amd import adi_io
+    nx, ny, nz, iters = 30, 26, 22, 4
+    path = tmp_path / "u.dat"
+    adi.main(["-nx", str(nx), "-ny", str(ny), "-nz", str(nz), "-iter", str(iters),
+              "-prec", str(prec), "-prof", "0", "-fused", str(fused), "-dump", str(path)])
+    dt = np.float64 if prec == 64 else np.float32
+    u = np.zeros((nz, ny, nx), dt)
+    u[0], u[-1] = 1, 1
+    u[:, 0], u[:, -1] = 1, 1
+    u[:, :, 0], u[:, :, -1] = 1, 1
+    u = u.reshape(-1)
+    oracle.adi(1.0, u, nx, ny, nz, iters)
+    ref = tmp_path / "ref.dat"
+    adi_io.dump_u(ref, u, nx, ny, nz)
+    r = adi_io.compare(ref, path, nx, ny, nz, prec)
+    assert r["nan_points"] == 0
+    assert r["normwise_rel"] <= (1e-12 if prec == 64 else 1e-5)

@@ -5,12 +5,15 @@ Library calls:
   tridAdiPreproc{D,S}  right-hand side and coefficients (apps/adi/src/preproc.cu:40-90)
   tridAdiStep{D,S}     preproc -> solve x -> solve y -> u += solve z
   tridAdiRun{D,S}      `iters` steps, optionally one captured hipGraph replayed
+  tridAdi{Step,Run}Fused{D,S}  matrix-free step: a, b, c computed in the sweeps
 
 Command line (same options as apps/adi/src/adi_cuda.cu:51-63,137-148):
   python -m tridsolver_amd.adi -nx 256 -ny 256 -nz 256 -iter 20 [-prec 32|64]
          [-optx 0 -opty 0 -optz 0] [-prof 1] [-graph 1] [-init ref|random]
+         [-fused 1] [-dump u.dat]
 Prints sum(u) and, with -prof 1, the time per grid element of each phase as
-the reference does (adi_cuda.cu:377-384).
+the reference does (adi_cuda.cu:377-384). -dump writes u in the reference
+apps' raw format (apps/adi/src/print_array.c:74-96; see adi_io.py).
 """
 from __future__ import annotations
 
@@ -135,6 +138,8 @@ def main(argv=None):
                           ("devid", 0)):
         ap.add_argument("-" + name, "--" + name, type=int, default=default)
     ap.add_argument("-init", "--init", default="ref", choices=("ref", "random"))
+    ap.add_argument("-fused", "--fused", type=int, default=0)
+    ap.add_argument("-dump", "--dump", default=None)
     a = ap.parse_args(argv)
     torch.cuda.set_device(a.devid)
     dtype = torch.float64 if a.prec == 64 else torch.float32
@@ -143,11 +148,22 @@ def main(argv=None):
     ws = Workspace(a.nx, a.ny, a.nz, dtype)
     lam = 1.0
     opts = (a.optx, a.opty, a.optz)
-    run(lam, u.clone(), ws, 1, graph=bool(a.graph), opts=opts, sync=True)  # warm-up
+    shape = (a.nx, a.ny, a.nz)
+
+    def go(uu, iters):
+        if a.fused:
+            run_fused(lam, uu, ws.du, iters, shape, graph=bool(a.graph), sync=True)
+        else:
+            run(lam, uu, ws, iters, graph=bool(a.graph), opts=opts, sync=True)
+
+    go(u.clone(), 1)  # warm-up
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    run(lam, u, ws, a.iter, graph=bool(a.graph), opts=opts, sync=True)
+    go(u, a.iter)
     total = time.perf_counter() - t0
+    if a.dump:
+        from .adi_io import dump_u
+        dump_u(a.dump, u, a.nx, a.ny, a.nz)
     print("\nComputing ADI on GPU: %f (s)" % total)
     print("end h_u sum = %.17g" % float(u.double().sum()))
     n = a.nx * a.ny * a.nz
