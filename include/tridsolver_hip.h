@@ -45,8 +45,8 @@ EXTERN_C size_t tridHipWorkspaceBytes(int fp64, int inc, int ndim, int solvedim,
 
 /* What the last solve of this thread ran: algorithm (1 per-lane Thomas,
  * 2 mixed precision, 3 hybrid, 4 adjoint, 5 ragged, 6 streamed strided
- * hybrid, 7 streamed strided hybrid with LDS storage, 8 wide strided hybrid),
- * and for the hybrid its
+ * hybrid, 7 streamed strided hybrid with LDS storage, 8 wide strided hybrid,
+ * 9 ragged mixed precision), and for the hybrid its
  * rows per lane M, lanes per system P, systems per workgroup W and XCD
  * grouping G (consecutive system groups dealt to one XCD; 1 = plain order). */
 EXTERN_C tridStatus_t tridHipLastConfig(int *algorithm, int *M, int *P, int *W, int *G);
@@ -156,6 +156,14 @@ EXTERN_C tridStatus_t tridSmtsvRaggedBatchInc(const TridParams *ctx, const float
                                               const float *b, const float *c, float *d,
                                               float *u, const int64_t *offsets, int64_t nsys,
                                               int64_t max_len);
+/* FP64 ragged batch by FP32 sweeps plus `refinements` (0..8) FP64
+ * iterative-refinement passes, in one pass over HBM (40 B per point, as the
+ * FP64 ragged solve). Lengths up to 4096; otherwise as tridDmtsvRaggedBatch.
+ * Forced rows per lane (tridHipForceRaggedConfig) 4 or 8 are honoured. */
+EXTERN_C tridStatus_t tridDmtsvRaggedBatchMixed(const TridParams *ctx, const double *a,
+                                                const double *b, const double *c, double *d,
+                                                const int64_t *offsets, int64_t nsys,
+                                                int64_t max_len, int refinements);
 EXTERN_C size_t tridHipRaggedWorkspaceBytes(int64_t nsys);
 /* Forces the ragged kernel's rows per lane (4, 8, or 16 at FP64) and
  * workgroups per CU for this thread's calls; M = 0 restores the tuned choice. */

@@ -319,3 +319,49 @@ def test_config_ragged_40k(dtype):
     check_ragged(off, a, b, c, d, td.cpu().numpy(), dtype)
     print("ragged 40k %s: %d unknowns, config %s" % (np.dtype(dtype).name, off[-1],
                                                       api.last_config()))
+
+
+@pytest.mark.parametrize("M", [0, 4, 8])
+def test_ragged_mixed_vs_oracle(M):
+    """FP64 ragged batch by FP32 sweeps + FP64 refinement: within 1e-12 of the
+    FP64 oracle after 2 refinements; 0 refinements is an FP32-accurate solve."""
+    rng = np.random.default_rng(40 + M)
+    max_len = 2048 if M == 4 else 4096
+    lens = np.round(np.exp(rng.uniform(np.log(1), np.log(max_len), 2000))).astype(np.int64)
+    lens[:5] = [0, 1, 2, 0, max_len]
+    off, a, b, c, d = ragged_case(rng, lens, np.float64)
+    ctx = api.forced_ragged(M) if M else None
+    if ctx:
+        ctx.__enter__()
+    try:
+        td = dev(d)
+        api.ragged_batch_mixed(dev(a), dev(b), dev(c), td, dev(off), max_len=max_len,
+                               refinements=2, sync=True)
+        assert api.last_config()[0] == 9
+        check_ragged(off, a, b, c, d, td.cpu().numpy(), np.float64)
+        t0 = dev(d)
+        api.ragged_batch_mixed(dev(a), dev(b), dev(c), t0, dev(off), max_len=max_len,
+                               refinements=0, sync=True)
+        want = d.copy()
+        oracle.ragged(a, b, c, want, off)
+        err = H.norm_rel_err(t0.cpu().numpy(), want)
+        assert 1e-12 < err <= 1e-5  # FP32 accuracy without refinement
+    finally:
+        if ctx:
+            ctx.__exit__(None, None, None)
+
+
+def test_ragged_mixed_rejects():
+    rng = np.random.default_rng(41)
+    off, a, b, c, d = ragged_case(rng, [10, 5000, 5], np.float64)
+    td = dev(d)
+    with pytest.raises(Exception):  # longer than 4096
+        api.ragged_batch_mixed(dev(a), dev(b), dev(c), td, dev(off), max_len=8192, sync=True)
+    off, a, b, c, d = ragged_case(rng, [10, 4000, 5], np.float64)
+    with pytest.raises(Exception):  # max_len promises less than a system needs
+        api.ragged_batch_mixed(dev(a), dev(b), dev(c), dev(d), dev(off), max_len=1000,
+                               sync=True)
+    with pytest.raises(TypeError):
+        api.ragged_batch_mixed(dev(a.astype(np.float32)), dev(b.astype(np.float32)),
+                               dev(c.astype(np.float32)), dev(d.astype(np.float32)), dev(off),
+                               max_len=4096)

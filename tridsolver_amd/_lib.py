@@ -70,6 +70,7 @@ SIGNATURES = {
     "tridDmtsvRaggedBatch": ([P] * 6 + [L, L], I),
     "tridSmtsvRaggedBatch": ([P] * 6 + [L, L], I),
     "tridDmtsvRaggedBatchInc": ([P] * 7 + [L, L], I),
+    "tridDmtsvRaggedBatchMixed": ([P] * 6 + [L, L, I], I),
     "tridSmtsvRaggedBatchInc": ([P] * 7 + [L, L], I),
     "tridHipRaggedWorkspaceBytes": ([L], SZ),
     "tridHipSetAutotuneMode": ([I], I),

@@ -176,6 +176,25 @@ def ragged_batch(a, b, c, d, offsets, *, max_len, u=None, sync=False, stream=Non
     return u if u is not None else d
 
 
+def ragged_batch_mixed(a, b, c, d, offsets, *, max_len, refinements=2, sync=False,
+                       stream=None):
+    """FP64 ragged batch by FP32 sweeps plus FP64 refinement
+    (tridDmtsvRaggedBatchMixed); lengths up to 4096."""
+    if d.dtype != torch.float64:
+        raise TypeError("the mixed-precision ragged solve takes FP64 arrays")
+    if offsets.dtype != torch.int64 or offsets.device != d.device:
+        raise TypeError("offsets must be an int64 tensor on the arrays' device")
+    arrs = [a, b, c, d]
+    _validate(arrs, [d.numel()], [d.numel()], 0)
+    use_torch_stream(stream)
+    params = make_params((0, 0, 0), 1 if sync else 0)
+    args = [_ptr(t) for t in arrs] + [_ptr(offsets)]
+    st = _lib.lib().tridDmtsvRaggedBatchMixed(C.byref(params), *args, offsets.numel() - 1,
+                                              int(max_len), int(refinements))
+    check(st, "tridDmtsvRaggedBatchMixed")
+    return d
+
+
 class forced_ragged:
     """Context manager forcing the ragged kernel's rows per lane and
     workgroups per CU for this thread's calls."""

@@ -20,6 +20,7 @@
 
 #include "../../include/tridsolver_hip.h"
 #include "hybrid_core.hpp"
+#include "mixed_core.hpp"
 #include "trid_internal.h"
 
 namespace trid {
@@ -157,11 +158,8 @@ __global__ __launch_bounds__((RgBounds<REAL, M>::threads)) void rg_solve(
   }
 }
 
-template <typename REAL, int M, bool INC>
-hipError_t launch_m(const REAL *a, const REAL *b, const REAL *c, REAL *d, REAL *u,
-                    const int64_t *off, int64_t S, int per_cu, void *ws, hipStream_t st,
-                    int dev) {
-  constexpr int Tmax = RgBounds<REAL, M>::threads;
+// The four planning launches: work items of at most Tmax lanes.
+void launch_plan(const int64_t *off, int64_t S, int M, int Tmax, void *ws, hipStream_t st) {
   RaggedPlan *plan = static_cast<RaggedPlan *>(ws);
   int *list = reinterpret_cast<int *>(static_cast<unsigned char *>(ws) + 512);
   hipLaunchKernelGGL(rg_reset, dim3(1), dim3(64), 0, st, plan);
@@ -173,6 +171,16 @@ hipError_t launch_m(const REAL *a, const REAL *b, const REAL *c, REAL *d, REAL *
   if (nb > 0) {
     hipLaunchKernelGGL(rg_scatter, dim3(nb), dim3(256), 0, st, off, S, M, Tmax, plan, list);
   }
+}
+
+template <typename REAL, int M, bool INC>
+hipError_t launch_m(const REAL *a, const REAL *b, const REAL *c, REAL *d, REAL *u,
+                    const int64_t *off, int64_t S, int per_cu, void *ws, hipStream_t st,
+                    int dev) {
+  constexpr int Tmax = RgBounds<REAL, M>::threads;
+  RaggedPlan *plan = static_cast<RaggedPlan *>(ws);
+  int *list = reinterpret_cast<int *>(static_cast<unsigned char *>(ws) + 512);
+  launch_plan(off, S, M, Tmax, ws, st);
   const size_t lds = sizeof(REAL) * (size_t)Tmax * (M + 1) + hyb::ring_bytes<REAL>(Tmax, 3);
   auto kern = rg_solve<REAL, M, INC>;
   static std::atomic<size_t> lds_granted{0};
@@ -187,7 +195,97 @@ hipError_t launch_m(const REAL *a, const REAL *b, const REAL *c, REAL *d, REAL *
   return hipGetLastError();
 }
 
+// FP64 systems by FP32 sweeps plus FP64 refinement (mixed_core.hpp): the
+// same work queue, 512-thread workgroups (P <= 2^MIXED_MAX_STEPS lanes).
+constexpr int RG_MIXED_THREADS = 512;
+
+template <int M>
+__global__ __launch_bounds__(RG_MIXED_THREADS) void rg_solve_mixed(
+    const double *__restrict__ a, const double *__restrict__ b, const double *__restrict__ c,
+    double *__restrict__ d, const int64_t *__restrict__ off, const int *__restrict__ list,
+    RaggedPlan *plan, int refine) {
+  constexpr int Tmax = RG_MIXED_THREADS;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ int s_item;
+  const int t = threadIdx.x;
+  const int total = plan->total_items;
+  for (;;) {
+    if (t == 0) s_item = atomicAdd(&plan->work, 1);
+    __syncthreads();
+    const int item = s_item;
+    __syncthreads();  // everyone has read s_item before it changes
+    if (item >= total) break;
+    int k = 0;
+    while (k + 1 < RG_CLASSES && plan->item_prefix[k + 1] <= item) ++k;
+    const int cls = RG_CLASSES - 1 - k;
+    hyb::Lane L;
+    L.P = 1 << cls;
+    L.W = Tmax >> cls;
+    L.T = Tmax;
+    L.t = t;
+    L.p = t % L.P;
+    L.w = t / L.P;
+    L.pstep = 1;
+    L.stride = 1;
+    const int64_t first = plan->sys_prefix[cls] + (int64_t)(item - plan->item_prefix[k]) * L.W;
+    const int64_t slot = first + L.w;
+    L.active = slot < plan->sys_prefix[cls + 1];
+    int64_t sys = L.active ? list[slot] : 0;
+    L.base = L.active ? off[sys] : 0;
+    L.N = L.active ? off[sys + 1] - off[sys] : 0;
+
+    double *tile = reinterpret_cast<double *>(smem_raw) + (size_t)L.w * L.P * (M + 1);
+    hyb::Ring ring{smem_raw + sizeof(double) * (size_t)Tmax * (M + 1),
+                   (int)(3 * sizeof(double) * Tmax), 0};
+    double A[M], B[M], Cc[M], D[M];
+    hyb::fetch_abcd<M, true>(L, a, b, c, d, A, B, Cc, D);
+    hyb::to_chunk<M, true>(L, A, tile);
+    hyb::to_chunk<M, true>(L, B, tile);
+    hyb::to_chunk<M, true>(L, Cc, tile);
+    hyb::to_chunk<M, true>(L, D, tile);
+    double X[M];
+    mixed_chunk<M>(L, ring, A, B, Cc, D, refine, X);
+    hyb::to_fetch<M, true>(L, X, tile);
+    hyb::store<M, true>(L, d, X);
+  }
+}
+
+template <int M>
+hipError_t launch_mixed_m(const double *a, const double *b, const double *c, double *d,
+                          const int64_t *off, int64_t S, int per_cu, int refine, void *ws,
+                          hipStream_t st, int dev) {
+  constexpr int Tmax = RG_MIXED_THREADS;
+  RaggedPlan *plan = static_cast<RaggedPlan *>(ws);
+  int *list = reinterpret_cast<int *>(static_cast<unsigned char *>(ws) + 512);
+  launch_plan(off, S, M, Tmax, ws, st);
+  const size_t lds =
+      sizeof(double) * (size_t)Tmax * (M + 1) + hyb::ring_bytes<double>(Tmax, 3);
+  auto kern = rg_solve_mixed<M>;
+  static std::atomic<size_t> lds_granted{0};
+  {
+    hipError_t e = ensure_lds(reinterpret_cast<const void *>(kern), lds, lds_granted);
+    if (e != hipSuccess) return e;
+  }
+  int cus = 256;
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  hipLaunchKernelGGL(kern, dim3((unsigned)(cus * std::max(1, per_cu))), dim3(Tmax), lds, st,
+                     a, b, c, d, off, list, plan, refine);
+  return hipGetLastError();
+}
+
 }  // namespace
+
+int ragged_mixed_max_len(int M) { return (M == 4 || M == 8) ? M * RG_MIXED_THREADS : 0; }
+
+hipError_t launch_ragged_mixed(int M, const double *a, const double *b, const double *c,
+                               double *d, const int64_t *off, int64_t S, int per_cu, int refine,
+                               void *ws, hipStream_t st, int dev) {
+  switch (M) {
+    case 4: return launch_mixed_m<4>(a, b, c, d, off, S, per_cu, refine, ws, st, dev);
+    case 8: return launch_mixed_m<8>(a, b, c, d, off, S, per_cu, refine, ws, st, dev);
+    default: return hipErrorInvalidValue;
+  }
+}
 
 size_t ragged_workspace_bytes(int64_t S) { return 512 + sizeof(int) * (size_t)std::max<int64_t>(S, 1); }
 

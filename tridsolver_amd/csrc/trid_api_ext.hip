@@ -44,6 +44,37 @@ tridStatus_t ragged(const TridParams *ctx, const REAL *a, const REAL *b, const R
   return s;
 }
 
+tridStatus_t ragged_mixed(const TridParams *ctx, const double *a, const double *b,
+                          const double *c, double *d, const int64_t *offsets, int64_t nsys,
+                          int64_t max_len, int refine) {
+  if (!a || !b || !c || !d || !offsets || nsys < 0 || max_len < 0 || refine < 0 || refine > 8)
+    return TRID_STATUS_INVALID_VALUE;
+  int dev = 0;
+  tridStatus_t s = check_device(&dev);
+  if (s != TRID_STATUS_SUCCESS) return s;
+  hipStream_t st = current_stream();
+  if (nsys == 0) return finish(ctx, st, hipSuccess);
+  // rows per lane: the forced value if it is a mixed one, else the smallest
+  // that covers max_len
+  int M = (tl_ragged_m == 4 || tl_ragged_m == 8) ? tl_ragged_m : (max_len <= 2048 ? 4 : 8);
+  const int supported = ragged_mixed_max_len(M);
+  if (supported == 0 || max_len > supported) return TRID_STATUS_INVALID_VALUE;
+  void *ws = nullptr;
+  if ((s = get_workspace(dev, st, ragged_workspace_bytes(nsys), &ws)) != TRID_STATUS_SUCCESS)
+    return s;
+  HybridCfg rec;
+  rec.M = M; rec.P = 0; rec.W = 0; rec.G = tl_ragged_per_cu;
+  record_config(9, &rec);
+  s = finish(ctx, st, launch_ragged_mixed(M, a, b, c, d, offsets, nsys, tl_ragged_per_cu,
+                                          refine, ws, st, dev));
+  if (s == TRID_STATUS_SUCCESS && (ctx == nullptr || ctx->sync != 0)) {
+    int flag = 0;
+    if (ragged_error_flag(ws, &flag) != hipSuccess) return TRID_STATUS_EXECUTION_FAILED;
+    if (flag) return TRID_STATUS_INVALID_VALUE;  // a system longer than supported
+  }
+  return s;
+}
+
 template <typename REAL>
 tridStatus_t adjoint(const TridParams *ctx, const REAL *a, const REAL *b, const REAL *c,
                      const REAL *x, const REAL *w, REAL *lam, REAL *ga, REAL *gb, REAL *gc,
@@ -161,3 +192,10 @@ EXTERN_C tridStatus_t tridHipForceRaggedConfig(int M, int per_cu) {
 }
 
 EXTERN_C size_t tridHipRaggedWorkspaceBytes(int64_t nsys) { return ragged_workspace_bytes(nsys); }
+
+EXTERN_C tridStatus_t tridDmtsvRaggedBatchMixed(const TridParams *ctx, const double *a,
+                                                const double *b, const double *c, double *d,
+                                                const int64_t *offsets, int64_t nsys,
+                                                int64_t max_len, int refinements) {
+  return ragged_mixed(ctx, a, b, c, d, offsets, nsys, max_len, refinements);
+}

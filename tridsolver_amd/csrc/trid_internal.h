@@ -96,6 +96,10 @@ hipError_t launch_adjoint(const Geom &g, const HybridCfg &cfg, const REAL *a, co
                           REAL *gb, REAL *gc, hipStream_t st);
 
 // ---- ragged batch (kernels_ragged.hip) ------------------------------------
+int ragged_mixed_max_len(int M);
+hipError_t launch_ragged_mixed(int M, const double *a, const double *b, const double *c,
+                               double *d, const int64_t *off, int64_t S, int per_cu, int refine,
+                               void *ws, hipStream_t st, int dev);
 size_t ragged_workspace_bytes(int64_t S);
 int ragged_max_len(int fp64, int M);
 template <typename REAL, bool INC>
