@@ -176,4 +176,57 @@ EXTERN_C tridStatus_t tridHipForceRaggedConfig(int M, int per_cu);
  * quoted against and to calibrate hardware byte counters. */
 EXTERN_C tridStatus_t tridHipStreamCopy(const void *src, void *dst, size_t bytes);
 
+/* ---- systems split across shards ------------------------------------------
+ * No counterpart in the reference's single-GPU API; the role of its
+ * distributed solver (scalar/src/cuda/trid_cuda_mpi.cu:454-519), for callers
+ * that hold each system in G contiguous pieces (one per GPU). Shard `shard` of
+ * `nshards` holds rows [o_k, o_k + dims[solvedim]) of every system in its own
+ * arrays (same ndim, solvedim, other dims; its own pads). Every shard needs at
+ * least 2 rows. a[0] of shard 0 and c[N-1] of the last shard are ignored as in
+ * the plain solve; on the other shards they couple to the neighbour shard.
+ *
+ *   1. tridXmtsvStridedBatchPartBoundary: writes 6 reals per system to `bnd`,
+ *      laid out [6][nsys] (systems in the order of the non-solve dimensions,
+ *      first fastest), reading a, b, c, d (none is written);
+ *   2. the caller gathers the nshards blocks shard-major into `bnd_all`
+ *      (nshards * 6 * nsys reals; an allgather in a distributed run);
+ *   3. tridXmtsvStridedBatch[Inc]PartSolve: the shard's rows of the solution,
+ *      in d (or added to u; d then stays untouched), by the batched solve.
+ * Steps 1 and 3 follow ctx like the plain solve (sync, opts hints). */
+EXTERN_C tridStatus_t tridDmtsvStridedBatchPartBoundary(const TridParams *ctx, const double *a,
+                                                        const double *b, const double *c,
+                                                        const double *d, int ndim,
+                                                        int solvedim, const int *dims,
+                                                        const int *pads, int shard,
+                                                        int nshards, double *bnd);
+EXTERN_C tridStatus_t tridSmtsvStridedBatchPartBoundary(const TridParams *ctx, const float *a,
+                                                        const float *b, const float *c,
+                                                        const float *d, int ndim,
+                                                        int solvedim, const int *dims,
+                                                        const int *pads, int shard,
+                                                        int nshards, float *bnd);
+EXTERN_C tridStatus_t tridDmtsvStridedBatchPartSolve(const TridParams *ctx, const double *a,
+                                                     const double *b, const double *c,
+                                                     double *d, int ndim, int solvedim,
+                                                     const int *dims, const int *pads,
+                                                     int shard, int nshards,
+                                                     const double *bnd_all);
+EXTERN_C tridStatus_t tridSmtsvStridedBatchPartSolve(const TridParams *ctx, const float *a,
+                                                     const float *b, const float *c, float *d,
+                                                     int ndim, int solvedim, const int *dims,
+                                                     const int *pads, int shard, int nshards,
+                                                     const float *bnd_all);
+EXTERN_C tridStatus_t tridDmtsvStridedBatchIncPartSolve(const TridParams *ctx, const double *a,
+                                                        const double *b, const double *c,
+                                                        double *d, double *u, int ndim,
+                                                        int solvedim, const int *dims,
+                                                        const int *pads, int shard,
+                                                        int nshards, const double *bnd_all);
+EXTERN_C tridStatus_t tridSmtsvStridedBatchIncPartSolve(const TridParams *ctx, const float *a,
+                                                        const float *b, const float *c,
+                                                        float *d, float *u, int ndim,
+                                                        int solvedim, const int *dims,
+                                                        const int *pads, int shard,
+                                                        int nshards, const float *bnd_all);
+
 #endif

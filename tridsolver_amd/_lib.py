@@ -71,6 +71,12 @@ SIGNATURES = {
     "tridSmtsvRaggedBatch": ([P] * 6 + [L, L], I),
     "tridDmtsvRaggedBatchInc": ([P] * 7 + [L, L], I),
     "tridDmtsvRaggedBatchMixed": ([P] * 6 + [L, L, I], I),
+    "tridDmtsvStridedBatchPartBoundary": ([P] * 5 + [I, I, P, P, I, I, P], I),
+    "tridSmtsvStridedBatchPartBoundary": ([P] * 5 + [I, I, P, P, I, I, P], I),
+    "tridDmtsvStridedBatchPartSolve": ([P] * 5 + [I, I, P, P, I, I, P], I),
+    "tridSmtsvStridedBatchPartSolve": ([P] * 5 + [I, I, P, P, I, I, P], I),
+    "tridDmtsvStridedBatchIncPartSolve": ([P] * 6 + [I, I, P, P, I, I, P], I),
+    "tridSmtsvStridedBatchIncPartSolve": ([P] * 6 + [I, I, P, P, I, I, P], I),
     "tridSmtsvRaggedBatchInc": ([P] * 7 + [L, L], I),
     "tridHipRaggedWorkspaceBytes": ([L], SZ),
     "tridHipSetAutotuneMode": ([I], I),

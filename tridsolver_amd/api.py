@@ -90,6 +90,61 @@ def trid_strided_batch(a, b, c, d, *, solvedim, dims=None, pads=None, u=None,
     return u if inc else d
 
 
+def part_boundary(a, b, c, d, bnd, *, solvedim, shard, nshards, dims=None, pads=None,
+                  offset=0, opts=(0, 0, 0), sync=False, stream=None):
+    """Step 1 of a solve whose systems are split across shards
+    (tridXmtsvStridedBatchPartBoundary): 6 reals per system into bnd
+    ([6][nsys]). This shard's arrays hold its rows of every system."""
+    if dims is None:
+        dims = list(d.shape[::-1])
+    if pads is None:
+        pads = list(d.shape[::-1]) if d.dim() == len(dims) else list(dims)
+    _validate([a, b, c, d], dims, pads, offset)
+    nsys = 1
+    for i, dm in enumerate(dims):
+        if i != solvedim:
+            nsys *= dm
+    if bnd.dtype != d.dtype or bnd.device != d.device or bnd.numel() < 6 * nsys:
+        raise ValueError("bnd must hold 6 * nsys values of the arrays' dtype and device")
+    use_torch_stream(stream)
+    name = ("tridD" if d.dtype == torch.float64 else "tridS") + "mtsvStridedBatchPartBoundary"
+    params = make_params(opts, 1 if sync else 0)
+    st = getattr(_lib.lib(), name)(C.byref(params), *[_ptr(t, offset) for t in (a, b, c, d)],
+                                   len(dims), solvedim, int3(dims), int3(pads), shard, nshards,
+                                   _ptr(bnd))
+    check(st, name)
+    return bnd
+
+
+def part_solve(a, b, c, d, bnd_all, *, solvedim, shard, nshards, dims=None, pads=None,
+               u=None, offset=0, opts=(0, 0, 0), sync=False, stream=None):
+    """Step 3 (tridXmtsvStridedBatch[Inc]PartSolve): this shard's rows of the
+    solution, given every shard's boundary block gathered shard-major."""
+    if dims is None:
+        dims = list(d.shape[::-1])
+    if pads is None:
+        pads = list(d.shape[::-1]) if d.dim() == len(dims) else list(dims)
+    inc = u is not None
+    arrs = [a, b, c, d] + ([u] if inc else [])
+    _validate(arrs, dims, pads, offset)
+    nsys = 1
+    for i, dm in enumerate(dims):
+        if i != solvedim:
+            nsys *= dm
+    if (bnd_all.dtype != d.dtype or bnd_all.device != d.device
+            or bnd_all.numel() < 6 * nsys * nshards):
+        raise ValueError("bnd_all must hold nshards * 6 * nsys values")
+    use_torch_stream(stream)
+    name = (("tridD" if d.dtype == torch.float64 else "tridS")
+            + ("mtsvStridedBatchIncPartSolve" if inc else "mtsvStridedBatchPartSolve"))
+    params = make_params(opts, 1 if sync else 0)
+    st = getattr(_lib.lib(), name)(C.byref(params), *[_ptr(t, offset) for t in arrs],
+                                   len(dims), solvedim, int3(dims), int3(pads), shard, nshards,
+                                   _ptr(bnd_all))
+    check(st, name)
+    return u if inc else d
+
+
 def last_config():
     """(algorithm, M, P, W, G) of this thread's last solve (1 lane, 3 hybrid)."""
     v = [C.c_int(0) for _ in range(5)]

@@ -15,6 +15,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -37,7 +38,9 @@ struct Workspace {
   void *ptr = nullptr;
   size_t bytes = 0;
 };
-std::map<std::pair<int, hipStream_t>, Workspace> g_ws;
+// keyed by (device, stream, role): role 0 is the solve's and the ragged
+// plan's workspace, role 1 the partitioned solve's saved boundary rows
+std::map<std::tuple<int, hipStream_t, int>, Workspace> g_ws;
 // Buffers replaced by a larger one. A hipGraph captured earlier may still
 // point at them, so they are only freed by tridHipReleaseWorkspaces().
 std::map<int, std::vector<void *>> g_retired;
@@ -68,9 +71,9 @@ tridStatus_t check_device(int *dev_out) {
 // Returns a workspace of at least `bytes` for (device, stream). Grows it with
 // hipMalloc outside stream capture; inside capture a too-small workspace is
 // an error (TRID_STATUS_INTERNAL_ERROR) because allocation is not capturable.
-tridStatus_t get_workspace(int dev, hipStream_t st, size_t bytes, void **out) {
+tridStatus_t get_workspace(int dev, hipStream_t st, size_t bytes, void **out, int role) {
   std::lock_guard<std::mutex> lk(g_mu);
-  Workspace &ws = g_ws[std::make_pair(dev, st)];
+  Workspace &ws = g_ws[std::make_tuple(dev, st, role)];
   if (ws.bytes >= bytes) {
     *out = ws.ptr;
     return TRID_STATUS_SUCCESS;
@@ -192,6 +195,15 @@ tridStatus_t solve(const TridParams *ctx, const REAL *a, const REAL *b, const RE
   return finish(ctx, st, launch_lane<REAL, INC>(g, a, b, c, d, u, cws, dws, st));
 }
 
+#define TRID_SOLVE_INST(REAL, INC)                                                          \
+  template tridStatus_t solve<REAL, INC>(const TridParams *, const REAL *, const REAL *,   \
+                                         const REAL *, REAL *, REAL *, int, int,           \
+                                         const int *, const int *);
+TRID_SOLVE_INST(float, false)
+TRID_SOLVE_INST(float, true)
+TRID_SOLVE_INST(double, false)
+TRID_SOLVE_INST(double, true)
+
 }  // namespace trid
 
 using namespace trid;
@@ -262,7 +274,7 @@ EXTERN_C tridStatus_t tridHipReleaseWorkspaces(void) {
   for (void *p : g_retired[dev]) (void)hipFree(p);
   g_retired[dev].clear();
   for (auto it = g_ws.begin(); it != g_ws.end();) {
-    if (it->first.first == dev) {
+    if (std::get<0>(it->first) == dev) {
       (void)hipFree(it->second.ptr);
       it = g_ws.erase(it);
     } else {

@@ -61,9 +61,16 @@ hipError_t launch_hybrid(const Geom &g, const HybridCfg &cfg, const REAL *a,
                          hipStream_t st);
 
 // ---- shared host helpers (trid_api.hip) ---------------------------------
+// The batched solve behind tridXmtsvStridedBatch[Inc] (shape choice, launch,
+// ctx->sync).
+template <typename REAL, bool INC>
+tridStatus_t solve(const TridParams *ctx, const REAL *a, const REAL *b, const REAL *c,
+                   REAL *d, REAL *u, int ndim, int solvedim, const int *dims,
+                   const int *pads);
 hipStream_t current_stream();
 tridStatus_t check_device(int *dev);
-tridStatus_t get_workspace(int dev, hipStream_t st, size_t bytes, void **out);
+// role 0: the solve's / ragged plan's workspace; 1: partitioned-solve saves
+tridStatus_t get_workspace(int dev, hipStream_t st, size_t bytes, void **out, int role = 0);
 
 // ---- streamed strided variant (kernels_stream.hip) ------------------------
 template <typename REAL>
