@@ -415,8 +415,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(ach / PEAK_HBM_GBS, 4),
                      "traffic": traffic, "kernel": "solvedim %s (slowest direction)" % dom},
-        "cpu_baseline": None if cpu is None else {k: cpu[k] for k in
-                                                  ("value", "unit", "cores", "kind", "sample")},
+        "cpu_baseline": None if cpu is None else {
+            k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample", "alg_GBs",
+                                "per_dir_gpts_s", "nproc", "single_thread") if k in cpu},
         "per_direction": dirs,
         "copy_GBs": round(copy_bw, 1),
         "extras": extras,
