@@ -162,9 +162,8 @@ def _wide_shape(dtype, M, N, seg=256):
     per_wave = max(1, 64 // W)
     P = -(-N // M)
     P = -(-P // per_wave) * per_wave
-    threads = 512 if dtype == np.float64 else 1024
-    slots = 2 if seg == 256 else 1
-    lds = P * M * seg + slots * 3 * W * P * esize
+    threads = 512 if (dtype == np.float64 and M == 32) else 1024
+    lds = P * M * seg + 3 * W * P * esize  # the tile and a one-slot PCR ring
     if W * P > threads or lds > 160 * 1024:
         return None, W
     return P, W

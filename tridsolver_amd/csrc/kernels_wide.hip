@@ -17,8 +17,8 @@
 // Arithmetic is the hybrid's (hybrid_core.hpp phases (i)-(iv)); only where
 // one operand lives differs.
 //
-// LDS: tile P*M rows x 256 bytes, plus the PCR ring (2 x 3 values per
-// thread). 512 rows: 128 KiB + 24 KiB.
+// LDS: tile P*M rows x SEG bytes, plus the PCR ring (3 values per thread per
+// slot, two slots when they fit). 512 rows of 256 bytes: 128 KiB + 24 KiB.
 
 #include <algorithm>
 #include <atomic>
@@ -29,22 +29,38 @@
 namespace trid {
 namespace {
 
-// SEG: bytes of one row segment (W * sizeof(REAL)), 128 or 256. With 128-byte
-// segments a workgroup is small enough for two per CU, so the PCR exchange
-// uses a single LDS slot (one more barrier per step, half the ring).
+// SEG: bytes of one row segment (W * sizeof(REAL)), 128 or 256.
 template <typename REAL, int SEG>
 struct WideShape {
   static constexpr int W = SEG / (int)sizeof(REAL);
-  // FP64: 512 threads, 256 VGPRs per lane available; FP32: 1024 threads.
-  static constexpr int threads = sizeof(REAL) == 8 ? 512 : 1024;
-  static constexpr int ring_slots = SEG == 256 ? 2 : 1;
 };
+// Threads per workgroup the register file allows: FP64 M = 32 holds 3 x 64
+// VGPRs of b, c, d (512 threads); everything else fits 1024.
+template <typename REAL, int M>
+struct WideBounds {
+  static constexpr int threads = (sizeof(REAL) == 8 && M == 32) ? 512 : 1024;
+};
+constexpr size_t WIDE_LDS = 160 * 1024;
+
+// PCR exchange slots: one (two barriers per step) for 128-byte rows, whose
+// workgroups are small enough for two per CU only with half the ring; for
+// 256-byte rows two (one barrier per step) when the LDS allows.
+template <typename REAL>
+size_t wide_lds_bytes(int M, int P, int seg, int slots) {
+  const int T = seg / (int)sizeof(REAL) * P;
+  return (size_t)P * M * seg + (size_t)slots * 3 * sizeof(REAL) * (size_t)T;
+}
+template <typename REAL>
+int wide_slots(int M, int P, int seg) {
+  if (seg == 128) return 1;
+  return wide_lds_bytes<REAL>(M, P, seg, 2) <= WIDE_LDS ? 2 : 1;
+}
 
 typedef __attribute__((address_space(3))) void lds_void;
 
 template <typename REAL, int M, int SEG, bool INC>
-__global__ __launch_bounds__((WideShape<REAL, SEG>::threads)) void hybrid_wide_kernel(
-    Geom g, int P, int G, const REAL *__restrict__ a, const REAL *__restrict__ b,
+__global__ __launch_bounds__((WideBounds<REAL, M>::threads)) void hybrid_wide_kernel(
+    Geom g, int P, int G, int slots, const REAL *__restrict__ a, const REAL *__restrict__ b,
     const REAL *__restrict__ c, REAL *__restrict__ d, REAL *__restrict__ u) {
   constexpr int W = WideShape<REAL, SEG>::W;
   constexpr int K = 256 / SEG;  // rows per wave-wide global->LDS load
@@ -54,7 +70,7 @@ __global__ __launch_bounds__((WideShape<REAL, SEG>::threads)) void hybrid_wide_k
   const int64_t R = (int64_t)P * M;  // rows in the tile (>= N)
   REAL *tile = reinterpret_cast<REAL *>(smem_raw);
   hyb::Ring ring{smem_raw + sizeof(REAL) * (size_t)R * W, (int)(3 * sizeof(REAL) * T), 0,
-                 WideShape<REAL, SEG>::ring_slots};
+                 slots};
 
   // (0) a -> tile[row][0..W) by global->LDS loads. One wave instruction
   // writes 256 contiguous LDS bytes (lane l: bytes 4l..4l+3), i.e. K rows:
@@ -188,19 +204,13 @@ __global__ __launch_bounds__((WideShape<REAL, SEG>::threads)) void hybrid_wide_k
   }
 }
 
-template <typename REAL, int SEG>
-size_t wide_lds_bytes(int M, int P) {
-  const int T = WideShape<REAL, SEG>::W * P;
-  return (size_t)P * M * SEG +
-         (size_t)WideShape<REAL, SEG>::ring_slots * 3 * sizeof(REAL) * (size_t)T;
-}
-
 template <typename REAL, int M, int SEG, bool INC>
 hipError_t launch_wide_m(const Geom &g, const HybridCfg &cfg, const REAL *a, const REAL *b,
                          const REAL *c, REAL *d, REAL *u, hipStream_t st) {
   constexpr int W = WideShape<REAL, SEG>::W;
   const int T = W * cfg.P;
-  const size_t lds = wide_lds_bytes<REAL, SEG>(M, cfg.P);
+  const int slots = wide_slots<REAL>(M, cfg.P, SEG);
+  const size_t lds = wide_lds_bytes<REAL>(M, cfg.P, SEG, slots);
   auto kern = hybrid_wide_kernel<REAL, M, SEG, INC>;
   static std::atomic<size_t> lds_granted{0};
   {
@@ -208,8 +218,8 @@ hipError_t launch_wide_m(const Geom &g, const HybridCfg &cfg, const REAL *a, con
     if (e != hipSuccess) return e;
   }
   const int64_t blocks = (g.n0 + W - 1) / W * g.n1;
-  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(T), lds, st, g, cfg.P, cfg.G, a, b, c,
-                     d, u);
+  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(T), lds, st, g, cfg.P, cfg.G, slots, a,
+                     b, c, d, u);
   return hipGetLastError();
 }
 
@@ -227,14 +237,13 @@ bool wide_cfg_valid(const Geom &g, const HybridCfg &c) {
   if ((int64_t)c.P * c.M < g.N) return false;
   const int W = c.W;
   const int64_t T = (int64_t)W * c.P;
-  if (T > WideShape<REAL, 256>::threads || T % 64 != 0) return false;  // whole waves
+  const int maxT = c.M == 32 ? WideBounds<REAL, 32>::threads : WideBounds<REAL, 16>::threads;
+  if (T > maxT || T % 64 != 0) return false;  // whole waves
   // 32-bit byte offsets: every address a lane forms lies below 4 GiB
   const int64_t nb0 = (g.n0 + W - 1) / W * W;
   const int64_t last = (nb0 - 1) + (g.n1 - 1) * g.st1 + ((int64_t)c.P * c.M - 1) * g.stride;
   if ((last + 1) * (int64_t)sizeof(REAL) > (int64_t)0xFFFFFFFFll) return false;
-  const size_t lds = seg == 256 ? wide_lds_bytes<REAL, 256>(c.M, c.P)
-                                 : wide_lds_bytes<REAL, 128>(c.M, c.P);
-  return lds <= 160 * 1024;
+  return wide_lds_bytes<REAL>(c.M, c.P, seg, 1) <= WIDE_LDS;
 }
 
 template <typename REAL, bool INC>
