@@ -286,7 +286,8 @@ def bench_adi(torch, n=256, iters=20):
         out[key] = {"step_ms": round(step_ms, 4), "gpts_s": round(n ** 3 / step_ms / 1e6, 2),
                     "steps": iters,
                     "bytes_per_point_step": 168 if key == "fp64" else 84}
-        # matrix-free step: coefficients computed in the sweeps, 72 (36) B/pt
+        # matrix-free step: coefficients and right-hand side computed in the
+        # sweeps, 56 (28) B/pt
         du = ws.du
         del ws
         adi.run_fused(1.0, u, du, 2, (n, n, n), graph=True, sync=True)
@@ -294,7 +295,7 @@ def bench_adi(torch, n=256, iters=20):
         t0 = time.perf_counter()
         adi.run_fused(1.0, u, du, iters, (n, n, n), graph=True, sync=True)
         step_ms = (time.perf_counter() - t0) * 1e3 / iters
-        byts = 72 if key == "fp64" else 36
+        byts = 56 if key == "fp64" else 28
         out[key + "_fused"] = {"step_ms": round(step_ms, 4),
                                "gpts_s": round(n ** 3 / step_ms / 1e6, 2), "steps": iters,
                                "bytes_per_point_step": byts,

@@ -46,10 +46,10 @@ EXTERN_C tridStatus_t tridAdiRunS(const TridParams *ctx, float lambda, float *u,
 
 /* Matrix-free timestep (no counterpart in the reference; its app writes nine
  * coefficient arrays, apps/adi/src/preproc.cu:77-85, from values fixed by
- * position, :59-73). The ADI coefficients
- * depend only on the grid position, so preproc writes only du and each sweep
- * computes a, b, c from (i, j, k): 72 B per point per step at FP64 instead of
- * 168, same results bit for bit as tridAdiStep. */
+ * position, :59-73). The ADI coefficients depend only on the grid position,
+ * so each sweep computes a, b, c from (i, j, k), and the x sweep computes its
+ * right-hand side from u: 56 B per point per step at FP64 instead of 168,
+ * same results bit for bit as tridAdiStep. du is workspace. */
 EXTERN_C tridStatus_t tridAdiStepFusedD(double lambda, double *u, double *du, int nx, int ny,
                                         int nz);
 EXTERN_C tridStatus_t tridAdiStepFusedS(float lambda, float *u, float *du, int nx, int ny,

@@ -36,6 +36,16 @@ struct Coef {
   REAL *a, *b, *c;
 };
 
+// Right-hand side at an interior point, from u at the point (c) and its six
+// neighbours; the one expression both the preproc kernel and the x sweep
+// that computes its own right-hand side evaluate, so they agree bit for bit.
+template <typename REAL>
+__device__ __forceinline__ REAL adi_rhs(REAL lambda, REAL xm, REAL xp, REAL ym, REAL yp,
+                                        REAL zm, REAL zp, REAL c) {
+  // lambda * (u[-1] + u[+1] + u[-nx] + u[+nx] + u[-nx*ny] + u[+nx*ny] - 6 u)
+  return lambda * (xm + xp + ym + yp + zm + zp - REAL(6) * c);
+}
+
 template <typename REAL>
 __global__ __launch_bounds__(PRE_BX *PRE_BY) void adi_preproc_kernel(
     REAL lambda, const REAL *__restrict__ u, REAL *__restrict__ du, Coef<REAL> c0,
@@ -56,9 +66,7 @@ __global__ __launch_bounds__(PRE_BX *PRE_BY) void adi_preproc_kernel(
     if (!edge_ij && k < nz - 1) up = u[ind + sz];
     REAL d, a, b, c;
     if (inner) {
-      // lambda * (u[-1] + u[+1] + u[-nx] + u[+nx] + u[-nx*ny] + u[+nx*ny] - 6 u)
-      d = lambda * (u[ind - sx] + u[ind + sx] + u[ind - sy] + u[ind + sy] + um + up -
-                    REAL(6) * uc);
+      d = adi_rhs(lambda, u[ind - sx], u[ind + sx], u[ind - sy], u[ind + sy], um, up, uc);
       a = ca;
       b = cb;
       c = ca;
@@ -152,17 +160,55 @@ struct AdiBounds {  // same register budget as the array-based hybrid kernel
   static constexpr int threads = sizeof(REAL) * M > 64 ? 512 : 1024;
 };
 
-template <typename REAL, int M, bool INC, bool CONTIG>
-__global__ __launch_bounds__((AdiBounds<REAL, M>::threads)) void adi_solve_kernel(Geom g, int P, int W, int G,
-                                                         AdiCoef<REAL> kc, REAL *__restrict__ d,
-                                                         REAL *__restrict__ u) {
+// RHS (x sweep only): the right-hand side is not read from d but computed
+// from u, as the preproc kernel does: the line's own u through the LDS tile
+// for the x neighbours, the four neighbour lines (j +- 1, k +- 1) loaded
+// alongside (coalesced; at 256^3 u stays in the last-level cache). d is
+// written only.
+template <typename REAL, int M, bool INC, bool CONTIG, bool RHS>
+__global__ __launch_bounds__((AdiBounds<REAL, M>::threads)) void adi_solve_kernel(
+    Geom g, int P, int W, int G, AdiCoef<REAL> kc, REAL *__restrict__ d,
+    REAL *__restrict__ u) {
+  static_assert(!RHS || (CONTIG && !INC), "right-hand side from u: x sweep only");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const hyb::Lane L = hyb::lane_of<CONTIG>(g, P, W, G);
   const size_t tile_bytes = CONTIG ? sizeof(REAL) * (size_t)W * P * (M + 1) : 0;
   REAL *tile = reinterpret_cast<REAL *>(smem_raw) + (size_t)L.w * P * (M + 1);
   hyb::Ring ring{smem_raw + tile_bytes, (int)(3 * sizeof(REAL) * L.T), 0};
   REAL ra[M], rb[M], rc[M], rd[M];
-  hyb::fetch<M, CONTIG>(L, d, rd, REAL(0), 0, 0, L.N);
+  if constexpr (RHS) {
+    REAL uc[M], ym[M], yp[M], zm[M], zp[M];
+    hyb::fetch<M, true>(L, u, uc, REAL(0), 0, 0, L.N);
+    const int j = (int)L.i0, k = (int)L.i1;  // the x line (j, k)
+    const bool line_inner = L.active && j > 0 && j < kc.ny - 1 && k > 0 && k < kc.nz - 1;
+    const int64_t sy = kc.nx, sz = (int64_t)kc.nx * kc.ny;
+#pragma unroll
+    for (int r = 0; r < M; ++r) {
+      const int64_t row = hyb::fetch_row<M, true>(L, r);
+      const bool ok = line_inner && row < L.N;
+      const int64_t o = L.base + row;
+      ym[r] = ok ? u[o - sy] : REAL(0);
+      yp[r] = ok ? u[o + sy] : REAL(0);
+      zm[r] = ok ? u[o - sz] : REAL(0);
+      zp[r] = ok ? u[o + sz] : REAL(0);
+    }
+    // x neighbours: the line's u in the tile, row order
+#pragma unroll
+    for (int r = 0; r < M; ++r) tile[hyb::skew<M>(L.p + L.P * r)] = uc[r];
+    __syncthreads();
+    const int last = P * M - 1;
+#pragma unroll
+    for (int r = 0; r < M; ++r) {
+      const int row = L.p + L.P * r;
+      const bool inner = line_inner && row > 0 && row < L.N - 1;
+      const REAL xm = tile[hyb::skew<M>(row > 0 ? row - 1 : 0)];
+      const REAL xp = tile[hyb::skew<M>(row < last ? row + 1 : last)];
+      rd[r] = inner ? adi_rhs(kc.lambda, xm, xp, ym[r], yp[r], zm[r], zp[r], uc[r]) : REAL(0);
+    }
+    __syncthreads();  // the tile is reused below
+  } else {
+    hyb::fetch<M, CONTIG>(L, d, rd, REAL(0), 0, 0, L.N);
+  }
   hyb::to_chunk<M, CONTIG>(L, rd, tile);
   adi_coefs<REAL, M, CONTIG>(L, kc, ra, rb, rc);
   REAL x[M];
@@ -172,13 +218,13 @@ __global__ __launch_bounds__((AdiBounds<REAL, M>::threads)) void adi_solve_kerne
   else hyb::store<M, CONTIG>(L, d, x);
 }
 
-template <typename REAL, int M, bool INC, bool CONTIG>
+template <typename REAL, int M, bool INC, bool CONTIG, bool RHS = false>
 hipError_t launch_adi_solve_m(const Geom &g, const HybridCfg &cfg, const AdiCoef<REAL> &kc,
                               REAL *d, REAL *u, hipStream_t st) {
   const int T = cfg.W * cfg.P;
   size_t lds = hyb::ring_bytes<REAL>(T, 3);
   if (CONTIG) lds += sizeof(REAL) * (size_t)T * (M + 1);
-  auto kern = adi_solve_kernel<REAL, M, INC, CONTIG>;
+  auto kern = adi_solve_kernel<REAL, M, INC, CONTIG, RHS>;
   static std::atomic<size_t> lds_granted{0};
   {
     hipError_t e = ensure_lds(reinterpret_cast<const void *>(kern), lds, lds_granted);
@@ -211,9 +257,10 @@ bool adi_cfg(int dev, const Geom &g, HybridCfg *cfg) {
   return hybrid_cfg_valid<REAL>(g, *cfg) || hybrid_plain_cfg<REAL>(g, cfg);
 }
 
-template <typename REAL, bool INC>
+template <typename REAL, bool INC, bool RHS = false>
 tridStatus_t adi_sweep(int dev, REAL lambda, REAL *d, REAL *u, int nx, int ny, int nz,
                        int sd) {
+  static_assert(!RHS || !INC, "right-hand side from u: x sweep only");
   const int dims[3] = {nx, ny, nz};
   Geom g;
   bool empty = false;
@@ -225,9 +272,19 @@ tridStatus_t adi_sweep(int dev, REAL lambda, REAL *d, REAL *u, int nx, int ny, i
   hipStream_t st = current_stream();
   const bool contig = sd == 0;
   hipError_t e;
+  if (RHS && !contig) return TRID_STATUS_INTERNAL_ERROR;
+  // FP32 x lines at 16 rows per lane would spill with the right-hand side
+  // in the sweep: there preproc writes du and the plain sweep reads it
+  constexpr bool RHS16 = RHS && sizeof(REAL) == 8;
+  if (RHS && !RHS16 && cfg.M == 16) {
+    tridStatus_t sp = preproc<REAL>(lambda, u, d, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                    nullptr, nullptr, nullptr, nullptr, nx, ny, nz);
+    if (sp != TRID_STATUS_SUCCESS) return sp;
+  }
 #define TRID_ADI_CASE(MM)                                                              \
   case MM:                                                                             \
-    e = contig ? launch_adi_solve_m<REAL, MM, INC, true>(g, cfg, kc, d, u, st)         \
+    e = contig ? launch_adi_solve_m<REAL, MM, INC, true, (MM == 16 ? RHS16 : RHS)>(    \
+                     g, cfg, kc, d, u, st)                                             \
                : launch_adi_solve_m<REAL, MM, INC, false>(g, cfg, kc, d, u, st);        \
     break;
   switch (cfg.M) {
@@ -247,16 +304,16 @@ tridStatus_t adi_sweep(int dev, REAL lambda, REAL *d, REAL *u, int nx, int ny, i
   return TRID_STATUS_SUCCESS;
 }
 
-// Matrix-free step: du = preproc(u); du = solve_x(du); du = solve_y(du);
-// u += solve_z(du).
+// Matrix-free step: du = solve_x(rhs(u)) (the right-hand side computed in
+// the x sweep); du = solve_y(du); u += solve_z(du). 56 B per point at FP64:
+// u read + du written, du read + written, du and u read + u written.
 template <typename REAL>
 tridStatus_t step_fused(REAL lambda, REAL *u, REAL *du, int nx, int ny, int nz) {
   int dev = 0;
   tridStatus_t s = check_device(&dev);
+  if (s == TRID_STATUS_SUCCESS && (nx < 1 || ny < 1 || nz < 1)) s = TRID_STATUS_INVALID_VALUE;
   if (s == TRID_STATUS_SUCCESS)
-    s = preproc<REAL>(lambda, u, du, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-                      nullptr, nullptr, nullptr, nx, ny, nz);
-  if (s == TRID_STATUS_SUCCESS) s = adi_sweep<REAL, false>(dev, lambda, du, u, nx, ny, nz, 0);
+    s = adi_sweep<REAL, false, true>(dev, lambda, du, u, nx, ny, nz, 0);
   if (s == TRID_STATUS_SUCCESS) s = adi_sweep<REAL, false>(dev, lambda, du, u, nx, ny, nz, 1);
   if (s == TRID_STATUS_SUCCESS) s = adi_sweep<REAL, true>(dev, lambda, du, u, nx, ny, nz, 2);
   return s;
