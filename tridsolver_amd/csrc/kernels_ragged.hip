@@ -5,10 +5,11 @@
 //
 // Plan (four tiny kernels, all stream-ordered and capturable):
 //   reset     zero the class counters, the work counter and the error flag
-//   classify  class(s) = log2 of the lanes a system needs, P = next_pow2(ceil(L/M))
+//   classify  class(s) = log2 of the lanes a system needs, P = next_pow2(ceil(L/M));
+//             counted per workgroup in LDS, one global atomic per class
 //   prefix    per class: first list slot and first work item; items are
 //             ordered longest class first (largest-first scheduling)
-//   scatter   list[slot] = s
+//   scatter   list[slot] = s, slots reserved per workgroup and class
 // Solve: a grid of one workgroup per CU (times `per_cu`); each workgroup
 // repeatedly takes the next work item from an atomic counter -- W = T/P
 // systems of one class, P lanes each, the Thomas-PCR hybrid of
@@ -60,13 +61,26 @@ __device__ __forceinline__ int rg_class(int64_t L, int M, int Tmax) {
   return c;
 }
 
-__global__ void rg_classify(const int64_t *__restrict__ off, int64_t S, int M, int Tmax,
-                            RaggedPlan *plan) {
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= S) return;
-  const int c = rg_class(off[s + 1] - off[s], M, Tmax);
-  if (c >= 0) atomicAdd(&plan->counts[c], 1);
-  else if (c == -2) atomicOr(&plan->error, 1);
+// Counting and scattering aggregate per workgroup in LDS first: one global
+// atomic per class and workgroup instead of one per system (systems of one
+// class would otherwise serialise on a single counter).
+constexpr int RG_PLAN_BLOCK = 256;
+
+__global__ __launch_bounds__(RG_PLAN_BLOCK) void rg_classify(const int64_t *__restrict__ off,
+                                                             int64_t S, int M, int Tmax,
+                                                             RaggedPlan *plan) {
+  __shared__ int cnt[RG_CLASSES];
+  const int t = threadIdx.x;
+  if (t < RG_CLASSES) cnt[t] = 0;
+  __syncthreads();
+  const int64_t s = (int64_t)blockIdx.x * RG_PLAN_BLOCK + t;
+  if (s < S) {
+    const int c = rg_class(off[s + 1] - off[s], M, Tmax);
+    if (c >= 0) atomicAdd(&cnt[c], 1);
+    else if (c == -2) atomicOr(&plan->error, 1);
+  }
+  __syncthreads();
+  if (t < RG_CLASSES && cnt[t] > 0) atomicAdd(&plan->counts[t], cnt[t]);
 }
 
 __global__ void rg_prefix(RaggedPlan *plan, int Tmax) {
@@ -89,14 +103,69 @@ __global__ void rg_prefix(RaggedPlan *plan, int Tmax) {
   plan->total_items = (int)ip;
 }
 
-__global__ void rg_scatter(const int64_t *__restrict__ off, int64_t S, int M, int Tmax,
-                           RaggedPlan *plan, int *__restrict__ list) {
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= S) return;
-  const int c = rg_class(off[s + 1] - off[s], M, Tmax);
-  if (c < 0) return;
-  const int pos = atomicAdd(&plan->cursor[c], 1);
-  list[plan->sys_prefix[c] + pos] = (int)s;
+__global__ __launch_bounds__(RG_PLAN_BLOCK) void rg_scatter(const int64_t *__restrict__ off,
+                                                            int64_t S, int M, int Tmax,
+                                                            RaggedPlan *plan,
+                                                            int *__restrict__ list) {
+  __shared__ int cnt[RG_CLASSES], first[RG_CLASSES];
+  const int t = threadIdx.x;
+  if (t < RG_CLASSES) cnt[t] = 0;
+  __syncthreads();
+  const int64_t s = (int64_t)blockIdx.x * RG_PLAN_BLOCK + t;
+  int c = -1, pos = 0;
+  if (s < S) {
+    c = rg_class(off[s + 1] - off[s], M, Tmax);
+    if (c >= 0) pos = atomicAdd(&cnt[c], 1);  // position among this block's class-c systems
+  }
+  __syncthreads();
+  if (t < RG_CLASSES && cnt[t] > 0) first[t] = atomicAdd(&plan->cursor[t], cnt[t]);
+  __syncthreads();
+  if (c >= 0) list[plan->sys_prefix[c] + first[c] + pos] = (int)s;
+}
+
+// One lane's view of work item `item`: the class (P lanes per system, W
+// systems), and the lane's system (offset, length) if it has one. The list
+// and offset loads depend on each other, so the persistent loops below decode
+// the next item while the current one is being solved.
+struct RgMeta {
+  int cls;
+  bool active;
+  int64_t base, N;
+};
+
+__device__ __forceinline__ RgMeta rg_meta(const RaggedPlan *plan, const int *__restrict__ list,
+                                          const int64_t *__restrict__ off, int item, int t,
+                                          int Tmax) {
+  RgMeta m;
+  int k = 0;
+  while (k + 1 < RG_CLASSES && plan->item_prefix[k + 1] <= item) ++k;
+  m.cls = RG_CLASSES - 1 - k;
+  const int P = 1 << m.cls, W = Tmax >> m.cls;
+  const int64_t first = plan->sys_prefix[m.cls] + (int64_t)(item - plan->item_prefix[k]) * W;
+  const int64_t slot = first + t / P;
+  m.active = slot < plan->sys_prefix[m.cls + 1];
+  const int64_t sys = m.active ? list[slot] : 0;
+  m.base = m.active ? off[sys] : 0;
+  m.N = m.active ? off[sys + 1] - off[sys] : 0;
+  return m;
+}
+
+__device__ __forceinline__ hyb::Lane rg_lane(const RgMeta &m, int t, int Tmax) {
+  hyb::Lane L;
+  L.P = 1 << m.cls;
+  L.W = Tmax >> m.cls;
+  L.T = Tmax;
+  L.t = t;
+  L.p = t % L.P;
+  L.w = t / L.P;
+  L.pstep = 1;
+  L.stride = 1;
+  L.active = m.active;
+  L.base = m.base;
+  L.N = m.N;
+  L.i0 = 0;
+  L.i1 = 0;
+  return L;
 }
 
 // Threads per workgroup: the register file holds 1024 threads of the M = 4/8
@@ -113,40 +182,27 @@ __global__ __launch_bounds__((RgBounds<REAL, M>::threads)) void rg_solve(
     const int *__restrict__ list, RaggedPlan *plan) {
   constexpr int Tmax = RgBounds<REAL, M>::threads;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  __shared__ int s_item;
+  __shared__ int s_next;
   const int t = threadIdx.x;
   const int total = plan->total_items;
+  if (t == 0) s_next = atomicAdd(&plan->work, 1);
+  __syncthreads();
+  int item = s_next;
+  RgMeta cur = item < total ? rg_meta(plan, list, off, item, t, Tmax) : RgMeta{};
   for (;;) {
-    if (t == 0) s_item = atomicAdd(&plan->work, 1);
-    __syncthreads();
-    const int item = s_item;
-    __syncthreads();  // everyone has read s_item before it changes
+    __syncthreads();  // every thread has read s_next
     if (item >= total) break;
-    int k = 0;
-    while (k + 1 < RG_CLASSES && plan->item_prefix[k + 1] <= item) ++k;
-    const int cls = RG_CLASSES - 1 - k;
-    hyb::Lane L;
-    L.P = 1 << cls;
-    L.W = Tmax >> cls;
-    L.T = Tmax;
-    L.t = t;
-    L.p = t % L.P;
-    L.w = t / L.P;
-    L.pstep = 1;
-    L.stride = 1;
-    const int64_t first = plan->sys_prefix[cls] + (int64_t)(item - plan->item_prefix[k]) * L.W;
-    const int64_t slot = first + L.w;
-    L.active = slot < plan->sys_prefix[cls + 1];
-    int64_t sys = L.active ? list[slot] : 0;
-    L.base = L.active ? off[sys] : 0;
-    L.N = L.active ? off[sys + 1] - off[sys] : 0;
-
+    // claim the following item now; its id is read after the next barrier
+    if (t == 0) s_next = atomicAdd(&plan->work, 1);
+    const hyb::Lane L = rg_lane(cur, t, Tmax);
     REAL *tile = reinterpret_cast<REAL *>(smem_raw) + (size_t)L.w * L.P * (M + 1);
     hyb::Ring ring{smem_raw + sizeof(REAL) * (size_t)Tmax * (M + 1),
                    (int)(3 * sizeof(REAL) * Tmax), 0};
     REAL ra[M], rb[M], rc[M], rd[M];
     hyb::fetch_abcd<M, true>(L, a, b, c, d, ra, rb, rc, rd);
-    hyb::to_chunk<M, true>(L, ra, tile);
+    hyb::to_chunk<M, true>(L, ra, tile);  // has barriers: s_next is visible below
+    const int nxt = s_next;
+    const RgMeta next = nxt < total ? rg_meta(plan, list, off, nxt, t, Tmax) : RgMeta{};
     hyb::to_chunk<M, true>(L, rb, tile);
     hyb::to_chunk<M, true>(L, rc, tile);
     hyb::to_chunk<M, true>(L, rd, tile);
@@ -155,6 +211,8 @@ __global__ __launch_bounds__((RgBounds<REAL, M>::threads)) void rg_solve(
     hyb::to_fetch<M, true>(L, x, tile);
     if (INC) hyb::store_add<M, true>(L, u, x);
     else hyb::store<M, true>(L, d, x);
+    item = nxt;
+    cur = next;
   }
 }
 
@@ -163,13 +221,15 @@ void launch_plan(const int64_t *off, int64_t S, int M, int Tmax, void *ws, hipSt
   RaggedPlan *plan = static_cast<RaggedPlan *>(ws);
   int *list = reinterpret_cast<int *>(static_cast<unsigned char *>(ws) + 512);
   hipLaunchKernelGGL(rg_reset, dim3(1), dim3(64), 0, st, plan);
-  const unsigned nb = (unsigned)((S + 255) / 256);
+  const unsigned nb = (unsigned)((S + RG_PLAN_BLOCK - 1) / RG_PLAN_BLOCK);
   if (nb > 0) {
-    hipLaunchKernelGGL(rg_classify, dim3(nb), dim3(256), 0, st, off, S, M, Tmax, plan);
+    hipLaunchKernelGGL(rg_classify, dim3(nb), dim3(RG_PLAN_BLOCK), 0, st, off, S, M, Tmax,
+                       plan);
   }
   hipLaunchKernelGGL(rg_prefix, dim3(1), dim3(64), 0, st, plan, Tmax);
   if (nb > 0) {
-    hipLaunchKernelGGL(rg_scatter, dim3(nb), dim3(256), 0, st, off, S, M, Tmax, plan, list);
+    hipLaunchKernelGGL(rg_scatter, dim3(nb), dim3(RG_PLAN_BLOCK), 0, st, off, S, M, Tmax,
+                       plan, list);
   }
 }
 
@@ -206,40 +266,26 @@ __global__ __launch_bounds__(RG_MIXED_THREADS) void rg_solve_mixed(
     RaggedPlan *plan, int refine) {
   constexpr int Tmax = RG_MIXED_THREADS;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  __shared__ int s_item;
+  __shared__ int s_next;
   const int t = threadIdx.x;
   const int total = plan->total_items;
+  if (t == 0) s_next = atomicAdd(&plan->work, 1);
+  __syncthreads();
+  int item = s_next;
+  RgMeta cur = item < total ? rg_meta(plan, list, off, item, t, Tmax) : RgMeta{};
   for (;;) {
-    if (t == 0) s_item = atomicAdd(&plan->work, 1);
-    __syncthreads();
-    const int item = s_item;
-    __syncthreads();  // everyone has read s_item before it changes
+    __syncthreads();  // every thread has read s_next
     if (item >= total) break;
-    int k = 0;
-    while (k + 1 < RG_CLASSES && plan->item_prefix[k + 1] <= item) ++k;
-    const int cls = RG_CLASSES - 1 - k;
-    hyb::Lane L;
-    L.P = 1 << cls;
-    L.W = Tmax >> cls;
-    L.T = Tmax;
-    L.t = t;
-    L.p = t % L.P;
-    L.w = t / L.P;
-    L.pstep = 1;
-    L.stride = 1;
-    const int64_t first = plan->sys_prefix[cls] + (int64_t)(item - plan->item_prefix[k]) * L.W;
-    const int64_t slot = first + L.w;
-    L.active = slot < plan->sys_prefix[cls + 1];
-    int64_t sys = L.active ? list[slot] : 0;
-    L.base = L.active ? off[sys] : 0;
-    L.N = L.active ? off[sys + 1] - off[sys] : 0;
-
+    if (t == 0) s_next = atomicAdd(&plan->work, 1);
+    const hyb::Lane L = rg_lane(cur, t, Tmax);
     double *tile = reinterpret_cast<double *>(smem_raw) + (size_t)L.w * L.P * (M + 1);
     hyb::Ring ring{smem_raw + sizeof(double) * (size_t)Tmax * (M + 1),
                    (int)(3 * sizeof(double) * Tmax), 0};
     double A[M], B[M], Cc[M], D[M];
     hyb::fetch_abcd<M, true>(L, a, b, c, d, A, B, Cc, D);
     hyb::to_chunk<M, true>(L, A, tile);
+    const int nxt = s_next;
+    const RgMeta next = nxt < total ? rg_meta(plan, list, off, nxt, t, Tmax) : RgMeta{};
     hyb::to_chunk<M, true>(L, B, tile);
     hyb::to_chunk<M, true>(L, Cc, tile);
     hyb::to_chunk<M, true>(L, D, tile);
@@ -247,6 +293,8 @@ __global__ __launch_bounds__(RG_MIXED_THREADS) void rg_solve_mixed(
     mixed_chunk<M>(L, ring, A, B, Cc, D, refine, X);
     hyb::to_fetch<M, true>(L, X, tile);
     hyb::store<M, true>(L, d, X);
+    item = nxt;
+    cur = next;
   }
 }
 
@@ -304,7 +352,8 @@ hipError_t launch_ragged(int M, const REAL *a, const REAL *b, const REAL *c, REA
     case 4: return launch_m<REAL, 4, INC>(a, b, c, d, u, off, S, per_cu, ws, st, dev);
     case 8: return launch_m<REAL, 8, INC>(a, b, c, d, u, off, S, per_cu, ws, st, dev);
     case 16:
-      if (sizeof(REAL) == 8) return launch_m<REAL, 16, INC>(a, b, c, d, u, off, S, per_cu, ws, st, dev);
+      if constexpr (sizeof(REAL) == 8)
+        return launch_m<REAL, 16, INC>(a, b, c, d, u, off, S, per_cu, ws, st, dev);
       return hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
   }
