@@ -271,6 +271,38 @@ def bench_ragged(torch, api, tune):
     return out
 
 
+def bench_partitioned(torch, api, n=512, shards=8):
+    """One GPU's share when each z system of an n^3 FP64 grid is split into
+    `shards` z-slabs (n x n x n/shards): the boundary step, the finishing
+    solve (given every shard's boundary block), and a plain local solve of
+    the same slab for comparison. The allgather between the two steps is not
+    timed (it needs the other GPUs); its size per GPU is reported."""
+    nz = n // shards
+    dims = [n, n, nz]
+    npts = n * n * nz
+    nsys = n * n
+    a, b, c, d = gen_gpu(torch, npts, torch.float64, seed=64)
+    bnd = torch.empty(6 * nsys, dtype=torch.float64, device="cuda")
+    k = shards // 2  # an interior shard: couplings on both sides
+    t_bnd = event_ms(torch, lambda: api.part_boundary(a, b, c, d, bnd, solvedim=2, shard=k,
+                                                      nshards=shards, dims=dims))
+    bnd_all = bnd.repeat(shards)  # stands in for the gathered blocks (timing only)
+    x = d.clone()
+    t_fin = event_ms(torch, lambda: api.part_solve(a, b, c, x, bnd_all, solvedim=2, shard=k,
+                                                   nshards=shards, dims=dims))
+    x = d.clone()
+    t_loc = event_ms(torch, lambda: api.trid_strided_batch(a, b, c, x, solvedim=2, dims=dims))
+    del a, b, c, d, x, bnd, bnd_all
+    torch.cuda.empty_cache()
+    return {"shard": dims, "shards": shards, "boundary_ms": round(t_bnd, 4),
+            # algorithmic bytes: a, b, c, d once (the bottom-up sweep's
+            # re-read of the same rows is served by the caches)
+            "boundary_alg_GBs": round(32.0 * npts / t_bnd / 1e6, 1),
+            "finish_ms": round(t_fin, 4), "local_solve_ms": round(t_loc, 4),
+            "partitioned_over_local": round((t_bnd + t_fin) / t_loc, 3),
+            "allgather_bytes_per_gpu": 6 * 8 * nsys * shards}
+
+
 def bench_adi(torch, n=256, iters=20):
     from tridsolver_amd import adi
     out = {}
@@ -391,6 +423,8 @@ def main():
         log("ragged: %s" % extras["ragged_40k"])
         extras["adi_256_graph"] = bench_adi(torch)
         log("adi: %s" % extras["adi_256_graph"])
+        extras["partitioned_z_512_8_shards"] = bench_partitioned(torch, api)
+        log("partitioned: %s" % extras["partitioned_z_512_8_shards"])
 
     cpu = None
     if not args.no_cpu:
