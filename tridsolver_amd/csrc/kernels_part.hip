@@ -250,12 +250,12 @@ tridStatus_t part_solve(const TridParams *ctx, const REAL *a, const REAL *b, con
   TridParams local = ctx ? *ctx : TridParams{nullptr, {0, 0, 0}, 1};
   local.sync = 0;
   s = solve<REAL, INC>(&local, a, b, c, d, u, ndim, solvedim, dims, pads);
-  if (s != TRID_STATUS_SUCCESS) return s;
-  if (INC) {
+  if (INC) {  // d back as it was, also when the solve failed
     hipLaunchKernelGGL(part_restore_kernel<REAL>, dim3(part_blocks(g)), dim3(PART_BLOCK), 0,
                        st, g, d, save);
     e = hipGetLastError();
   }
+  if (s != TRID_STATUS_SUCCESS) return s;
   return finish(ctx, st, e);
 }
 
