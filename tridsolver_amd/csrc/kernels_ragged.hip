@@ -10,11 +10,13 @@
 //   prefix    per class: first list slot and first work item; items are
 //             ordered longest class first (largest-first scheduling)
 //   scatter   list[slot] = s, slots reserved per workgroup and class
-// Solve: a grid of one workgroup per CU (times `per_cu`); each workgroup
-// repeatedly takes the next work item from an atomic counter -- W = T/P
-// systems of one class, P lanes each, the Thomas-PCR hybrid of
-// hybrid_core.hpp -- and leaves when the counter passes the last item, so every
-// wave exits. Systems longer than M*T set the error flag and are skipped.
+// Solve: two persistent launches. Systems needing more than T/2 lanes are
+// solved by one T-thread workgroup per CU (times `per_cu`), the rest by two
+// T/2-thread workgroups per CU. Each workgroup repeatedly takes the next work
+// item of its range from an atomic counter -- W systems of one class, P
+// lanes each, the Thomas-PCR hybrid of hybrid_core.hpp -- and leaves when the
+// counter passes the range's last item, so every wave exits. Systems longer
+// than M*T set the error flag and are skipped.
 
 #include <algorithm>
 #include <atomic>
@@ -34,9 +36,11 @@ struct RaggedPlan {
   int cursor[RG_CLASSES];
   int64_t sys_prefix[RG_CLASSES + 1];
   int64_t item_prefix[RG_CLASSES + 1];  // in scheduling order (longest first)
-  int work;
+  int work;       // next item of the long-system launch
+  int work_short; // next item of the short-system launch
   int error;
   int total_items;
+  int long_items;  // items [0, long_items) need more than Tshort lanes
 };
 
 __global__ void rg_reset(RaggedPlan *plan) {
@@ -47,6 +51,8 @@ __global__ void rg_reset(RaggedPlan *plan) {
   }
   if (t == 0) {
     plan->work = 0;
+    plan->work_short = 0;
+    plan->long_items = 0;
     plan->error = 0;
     plan->total_items = 0;
   }
@@ -83,7 +89,13 @@ __global__ __launch_bounds__(RG_PLAN_BLOCK) void rg_classify(const int64_t *__re
   if (t < RG_CLASSES && cnt[t] > 0) atomicAdd(&plan->counts[t], cnt[t]);
 }
 
-__global__ void rg_prefix(RaggedPlan *plan, int Tmax) {
+// Lanes of the workgroups that solve class c: Tmax for systems needing more
+// than Tshort lanes, Tshort for the rest.
+__host__ __device__ __forceinline__ int rg_threads(int c, int Tmax, int Tshort) {
+  return (1 << c) > Tshort ? Tmax : Tshort;
+}
+
+__global__ void rg_prefix(RaggedPlan *plan, int Tmax, int Tshort) {
   if (threadIdx.x != 0) return;
   int64_t sp = 0;
   for (int c = 0; c < RG_CLASSES; ++c) {
@@ -93,14 +105,18 @@ __global__ void rg_prefix(RaggedPlan *plan, int Tmax) {
   plan->sys_prefix[RG_CLASSES] = sp;
   // item_prefix[k] for scheduling slot k = class RG_CLASSES-1-k
   int64_t ip = 0;
+  int long_items = 0;
   for (int k = 0; k < RG_CLASSES; ++k) {
     const int c = RG_CLASSES - 1 - k;
     plan->item_prefix[k] = ip;
-    const int W = Tmax >> c;
-    ip += W > 0 ? (plan->counts[c] + W - 1) / W : 0;
+    const int W = rg_threads(c, Tmax, Tshort) >> c;
+    const int64_t items = W > 0 ? (plan->counts[c] + W - 1) / W : 0;
+    ip += items;
+    if ((1 << c) > Tshort) long_items += (int)items;
   }
   plan->item_prefix[RG_CLASSES] = ip;
   plan->total_items = (int)ip;
+  plan->long_items = long_items;
 }
 
 __global__ __launch_bounds__(RG_PLAN_BLOCK) void rg_scatter(const int64_t *__restrict__ off,
@@ -175,17 +191,23 @@ struct RgBounds {
   static constexpr int threads = sizeof(REAL) * M > 64 ? 512 : 1024;
 };
 
-template <typename REAL, int M, bool INC>
-__global__ __launch_bounds__((RgBounds<REAL, M>::threads)) void rg_solve(
+// One persistent launch over a range of items: LONG = the items whose systems
+// need more than Tshort lanes (workgroups of T = Tmax threads, one per CU),
+// else the rest (T = Tshort, two workgroups per CU, so that one loads while
+// the other computes).
+template <typename REAL, int M, bool INC, int T, bool LONG>
+__global__ __launch_bounds__(T) void rg_solve(
     const REAL *__restrict__ a, const REAL *__restrict__ b, const REAL *__restrict__ c,
     REAL *__restrict__ d, REAL *__restrict__ u, const int64_t *__restrict__ off,
     const int *__restrict__ list, RaggedPlan *plan) {
-  constexpr int Tmax = RgBounds<REAL, M>::threads;
+  constexpr int Tmax = T;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ int s_next;
   const int t = threadIdx.x;
-  const int total = plan->total_items;
-  if (t == 0) s_next = atomicAdd(&plan->work, 1);
+  int *counter = LONG ? &plan->work : &plan->work_short;
+  const int lo = LONG ? 0 : plan->long_items;
+  const int total = LONG ? plan->long_items : plan->total_items;
+  if (t == 0) s_next = lo + atomicAdd(counter, 1);
   __syncthreads();
   int item = s_next;
   RgMeta cur = item < total ? rg_meta(plan, list, off, item, t, Tmax) : RgMeta{};
@@ -193,7 +215,7 @@ __global__ __launch_bounds__((RgBounds<REAL, M>::threads)) void rg_solve(
     __syncthreads();  // every thread has read s_next
     if (item >= total) break;
     // claim the following item now; its id is read after the next barrier
-    if (t == 0) s_next = atomicAdd(&plan->work, 1);
+    if (t == 0) s_next = lo + atomicAdd(counter, 1);
     const hyb::Lane L = rg_lane(cur, t, Tmax);
     REAL *tile = reinterpret_cast<REAL *>(smem_raw) + (size_t)L.w * L.P * (M + 1);
     hyb::Ring ring{smem_raw + sizeof(REAL) * (size_t)Tmax * (M + 1),
@@ -217,7 +239,8 @@ __global__ __launch_bounds__((RgBounds<REAL, M>::threads)) void rg_solve(
 }
 
 // The four planning launches: work items of at most Tmax lanes.
-void launch_plan(const int64_t *off, int64_t S, int M, int Tmax, void *ws, hipStream_t st) {
+void launch_plan(const int64_t *off, int64_t S, int M, int Tmax, int Tshort, void *ws,
+                 hipStream_t st) {
   RaggedPlan *plan = static_cast<RaggedPlan *>(ws);
   int *list = reinterpret_cast<int *>(static_cast<unsigned char *>(ws) + 512);
   hipLaunchKernelGGL(rg_reset, dim3(1), dim3(64), 0, st, plan);
@@ -226,7 +249,7 @@ void launch_plan(const int64_t *off, int64_t S, int M, int Tmax, void *ws, hipSt
     hipLaunchKernelGGL(rg_classify, dim3(nb), dim3(RG_PLAN_BLOCK), 0, st, off, S, M, Tmax,
                        plan);
   }
-  hipLaunchKernelGGL(rg_prefix, dim3(1), dim3(64), 0, st, plan, Tmax);
+  hipLaunchKernelGGL(rg_prefix, dim3(1), dim3(64), 0, st, plan, Tmax, Tshort);
   if (nb > 0) {
     hipLaunchKernelGGL(rg_scatter, dim3(nb), dim3(RG_PLAN_BLOCK), 0, st, off, S, M, Tmax,
                        plan, list);
@@ -238,20 +261,30 @@ hipError_t launch_m(const REAL *a, const REAL *b, const REAL *c, REAL *d, REAL *
                     const int64_t *off, int64_t S, int per_cu, void *ws, hipStream_t st,
                     int dev) {
   constexpr int Tmax = RgBounds<REAL, M>::threads;
+  constexpr int Tshort = Tmax / 2;
   RaggedPlan *plan = static_cast<RaggedPlan *>(ws);
   int *list = reinterpret_cast<int *>(static_cast<unsigned char *>(ws) + 512);
-  launch_plan(off, S, M, Tmax, ws, st);
-  const size_t lds = sizeof(REAL) * (size_t)Tmax * (M + 1) + hyb::ring_bytes<REAL>(Tmax, 3);
-  auto kern = rg_solve<REAL, M, INC>;
-  static std::atomic<size_t> lds_granted{0};
-  {
-    hipError_t e = ensure_lds(reinterpret_cast<const void *>(kern), lds, lds_granted);
-    if (e != hipSuccess) return e;
-  }
+  launch_plan(off, S, M, Tmax, Tshort, ws, st);
   int cus = 256;
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  hipLaunchKernelGGL(kern, dim3((unsigned)(cus * std::max(1, per_cu))), dim3(Tmax), lds, st,
-                     a, b, c, d, u, off, list, plan);
+  const int per = std::max(1, per_cu);
+  {  // the long systems first (an empty range exits at once)
+    const size_t lds = sizeof(REAL) * (size_t)Tmax * (M + 1) + hyb::ring_bytes<REAL>(Tmax, 3);
+    auto kern = rg_solve<REAL, M, INC, Tmax, true>;
+    static std::atomic<size_t> lds_granted{0};
+    hipError_t e = ensure_lds(reinterpret_cast<const void *>(kern), lds, lds_granted);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kern, dim3((unsigned)(cus * per)), dim3(Tmax), lds, st, a, b, c, d, u,
+                       off, list, plan);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  const size_t lds = sizeof(REAL) * (size_t)Tshort * (M + 1) + hyb::ring_bytes<REAL>(Tshort, 3);
+  auto kern = rg_solve<REAL, M, INC, Tshort, false>;
+  static std::atomic<size_t> lds_granted{0};
+  hipError_t e = ensure_lds(reinterpret_cast<const void *>(kern), lds, lds_granted);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(kern, dim3((unsigned)(2 * cus * per)), dim3(Tshort), lds, st, a, b, c, d,
+                     u, off, list, plan);
   return hipGetLastError();
 }
 
@@ -305,7 +338,7 @@ hipError_t launch_mixed_m(const double *a, const double *b, const double *c, dou
   constexpr int Tmax = RG_MIXED_THREADS;
   RaggedPlan *plan = static_cast<RaggedPlan *>(ws);
   int *list = reinterpret_cast<int *>(static_cast<unsigned char *>(ws) + 512);
-  launch_plan(off, S, M, Tmax, ws, st);
+  launch_plan(off, S, M, Tmax, Tmax, ws, st);  // one launch: no long range
   const size_t lds =
       sizeof(double) * (size_t)Tmax * (M + 1) + hyb::ring_bytes<double>(Tmax, 3);
   auto kern = rg_solve_mixed<M>;
