@@ -188,17 +188,28 @@ def bench_mixed(torch, api, n, fp64_ms, tune):
             api.autotune_mixed(sd, [n, n, n])
         x64 = d0.clone()
         api.trid_strided_batch(a, b, c, x64, solvedim=sd, dims=[n, n, n])
-        xm = d0.clone()
-        api.mixed_strided_batch(a, b, c, xm, solvedim=sd, dims=[n, n, n], refinements=2)
-        err = float((xm - x64).abs().max() / x64.abs().max())
-        del xm, x64
-        d = d0.clone()
-        ms = event_ms(torch, lambda: api.mixed_strided_batch(a, b, c, d, solvedim=sd,
-                                                             dims=[n, n, n], refinements=2))
-        out[name] = {"ms": round(ms, 4), "gpts_s": round(n3 / ms / 1e6, 2),
-                     "speedup_vs_fp64": round(fp64_ms[name] / ms, 3),
-                     "max_rel_err_vs_fp64": err, "config": api.last_config()}
-        del d
+        runs = {}
+        for ref in (1, 2):  # FP64 refinement passes after the FP32 solve
+            xm = d0.clone()
+            api.mixed_strided_batch(a, b, c, xm, solvedim=sd, dims=[n, n, n], refinements=ref)
+            err = float((xm - x64).abs().max() / x64.abs().max())
+            del xm
+            d = d0.clone()
+            ms = event_ms(torch, lambda: api.mixed_strided_batch(a, b, c, d, solvedim=sd,
+                                                                 dims=[n, n, n],
+                                                                 refinements=ref))
+            del d
+            runs[ref] = {"ms": round(ms, 4), "speedup_vs_fp64": round(fp64_ms[name] / ms, 3),
+                         "max_rel_err_vs_fp64": err}
+        del x64
+        # the fewest refinements that meet 1e-12 against the FP64 solve
+        ref = 1 if runs[1]["max_rel_err_vs_fp64"] <= 1e-12 else 2
+        best = runs[ref]
+        out[name] = {"ms": best["ms"], "gpts_s": round(n3 / best["ms"] / 1e6, 2),
+                     "refinements": ref, "speedup_vs_fp64": best["speedup_vs_fp64"],
+                     "max_rel_err_vs_fp64": best["max_rel_err_vs_fp64"],
+                     "config": api.last_config(),
+                     "by_refinements": {str(k): v for k, v in runs.items()}}
     del a, b, c, d0
     torch.cuda.empty_cache()
     return out
