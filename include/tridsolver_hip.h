@@ -158,8 +158,10 @@ EXTERN_C tridStatus_t tridSmtsvRaggedBatchInc(const TridParams *ctx, const float
                                               int64_t max_len);
 /* FP64 ragged batch by FP32 sweeps plus `refinements` (0..8) FP64
  * iterative-refinement passes, in one pass over HBM (40 B per point, as the
- * FP64 ragged solve). Lengths up to 4096; otherwise as tridDmtsvRaggedBatch.
- * Forced rows per lane (tridHipForceRaggedConfig) 4 or 8 are honoured. */
+ * FP64 ragged solve). Lengths up to 8192 (16 rows per lane, a and c kept in
+ * LDS); otherwise as tridDmtsvRaggedBatch. Forced rows per lane
+ * (tridHipForceRaggedConfig) 4 or 8 keep every operand in registers and take
+ * lengths up to 2048 / 4096. */
 EXTERN_C tridStatus_t tridDmtsvRaggedBatchMixed(const TridParams *ctx, const double *a,
                                                 const double *b, const double *c, double *d,
                                                 const int64_t *offsets, int64_t nsys,

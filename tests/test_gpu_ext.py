@@ -76,6 +76,102 @@ def test_mixed_padded_layout():
         assert np.array_equal(got[~mask], arrs[3][~mask])
 
 
+def _mixed_lds_shape(dims, sd, W):
+    """A valid shape of the mixed kernel with a and c in LDS (M = 16):
+    contiguous systems W per workgroup; strided W x-neighbours (64- or
+    128-byte rows), P rounded to whole waves."""
+    N = dims[sd]
+    P = -(-N // 16)
+    if sd > 0:
+        P = -(-P // (64 // W)) * (64 // W)
+    return (16, P, W, 8, 3)
+
+
+@pytest.mark.parametrize("dims", [[64, 48, 40], [37, 21, 19], [9, 700, 2], [1000, 3]])
+def test_mixed_lds_variant(dims):
+    """The M = 16 kernels with a and c in LDS (contiguous tiles; strided
+    tiles filled by global->LDS loads), forced, against the long-double
+    oracle: 2 refinements <= 1e-12, 0 refinements FP32-accurate. Covers
+    partial workgroups (n0 not a multiple of W), lanes past N and both row
+    widths."""
+    rng = np.random.default_rng(sum(dims) + 3)
+    n = int(np.prod(dims))
+    a, b, c, d = H.diag_dominant(rng, n)
+    for sd in range(len(dims)):
+        xl = oracle.batch_ld(a, b, c, d, ndim=len(dims), solvedim=sd, dims=dims)
+        for W in ((1, 4) if sd == 0 else (8, 16)):
+            shape = _mixed_lds_shape(dims, sd, W)
+            if sd > 0 and shape[1] * W > 512:
+                continue  # longer than one 512-thread workgroup holds at this width
+            with api.forced_hybrid(*shape):
+                errs = []
+                for refine in (0, 2):
+                    td = dev(d)
+                    api.mixed_strided_batch(dev(a), dev(b), dev(c), td, solvedim=sd,
+                                            dims=dims, refinements=refine, sync=True)
+                    assert api.last_config()[1:4] == shape[:3]
+                    errs.append(H.norm_rel_err(td.cpu().numpy(), xl))
+            assert errs[0] <= 1e-5 and errs[1] <= 1e-12, (sd, W, errs)
+
+
+@pytest.mark.parametrize("M", [4, 8])
+def test_mixed_register_variant(M):
+    """The kernels with every operand in registers (M = 4, 8), forced, along
+    x, y and z: 2 refinements <= 1e-12 against the long-double oracle."""
+    rng = np.random.default_rng(90 + M)
+    dims = [40, 33, 50]
+    n = int(np.prod(dims))
+    a, b, c, d = H.diag_dominant(rng, n)
+    for sd in range(3):
+        xl = oracle.batch_ld(a, b, c, d, ndim=3, solvedim=sd, dims=dims)
+        P = -(-dims[sd] // M)
+        W = max(1, 256 // P) if sd == 0 else 4
+        with api.forced_hybrid(M, P, W, 8, 0):
+            td = dev(d)
+            api.mixed_strided_batch(dev(a), dev(b), dev(c), td, solvedim=sd, dims=dims,
+                                    refinements=2, sync=True)
+            assert api.last_config()[1:4] == (M, P, W)
+        assert H.norm_rel_err(td.cpu().numpy(), xl) <= 1e-12
+
+
+def test_mixed_lds_variant_padded():
+    """Halo-padded layout through the LDS variant: results match the oracle
+    and the halo is untouched."""
+    rng = np.random.default_rng(78)
+    dims = [30, 22, 14]
+    n = int(np.prod(dims))
+    a, b, c, d = H.diag_dominant(rng, n)
+    arrs = [H.pad_halo(t, dims) for t in (a, b, c, d)]
+    off, pads = H.halo_offset(dims), [t + 2 for t in dims]
+    for sd in range(3):
+        shape = _mixed_lds_shape(dims, sd, 4 if sd == 0 else 16)
+        td = dev(arrs[3])
+        with api.forced_hybrid(*shape):
+            api.mixed_strided_batch(*(dev(t) for t in arrs[:3]), td, solvedim=sd, dims=dims,
+                                    pads=pads, offset=off, refinements=2, sync=True)
+        od = arrs[3].copy()
+        oracle.batch(arrs[0], arrs[1], arrs[2], od, ndim=3, solvedim=sd, dims=dims, pads=pads,
+                     offset=off)
+        got = td.cpu().numpy()
+        H.assert_norm_close(got, od, np.float64)
+        mask = H.pad_halo(np.ones(n, dtype=bool), dims)
+        assert np.array_equal(got[~mask], arrs[3][~mask])
+
+
+def test_mixed_lds_rejects_unfit_shapes():
+    rng = np.random.default_rng(79)
+    dims = [64, 600, 2]
+    a, b, c, d = (dev(t) for t in H.diag_dominant(rng, int(np.prod(dims))))
+    bad = [(16, 40, 16, 8, 3),   # strided: 640 threads
+           (16, 38, 8, 8, 3),    # strided: 304 threads, not whole waves
+           (16, 40, 4, 8, 3),    # strided: 32-byte rows
+           (8, 76, 8, 8, 3)]     # LDS variant is built for M = 16 only
+    for shape in bad:
+        with api.forced_hybrid(*shape):
+            with pytest.raises(Exception):
+                api.mixed_strided_batch(a, b, c, d, solvedim=1, dims=dims, sync=True)
+
+
 # --------------------------------------------------------------------------
 # Adjoint
 # --------------------------------------------------------------------------
@@ -321,12 +417,29 @@ def test_config_ragged_40k(dtype):
                                                       api.last_config()))
 
 
-@pytest.mark.parametrize("M", [0, 4, 8])
+@pytest.mark.parametrize("refine", [1, 2])
+def test_config_ragged_40k_mixed(refine):
+    """The 40,000-system ragged config (lengths round(exp(U(ln 8, ln 8192))))
+    by FP32 sweeps + FP64 refinement, one persistent launch: within 1e-12 of
+    the FP64 oracle."""
+    rng = np.random.default_rng(40000)
+    lens = np.round(np.exp(rng.uniform(np.log(8), np.log(8192), 40000))).astype(np.int64)
+    off, a, b, c, d = ragged_case(rng, lens, np.float64)
+    td = dev(d)
+    api.ragged_batch_mixed(dev(a), dev(b), dev(c), td, dev(off), max_len=8192,
+                           refinements=refine, sync=True)
+    assert api.last_config()[:2] == (9, 16)
+    check_ragged(off, a, b, c, d, td.cpu().numpy(), np.float64)
+
+
+@pytest.mark.parametrize("M", [0, 4, 8, 16])
 def test_ragged_mixed_vs_oracle(M):
     """FP64 ragged batch by FP32 sweeps + FP64 refinement: within 1e-12 of the
-    FP64 oracle after 2 refinements; 0 refinements is an FP32-accurate solve."""
+    FP64 oracle after 2 refinements; 0 refinements is an FP32-accurate solve.
+    M = 4 and 8 keep every operand in registers, M = 16 (the default) keeps
+    a and c in LDS."""
     rng = np.random.default_rng(40 + M)
-    max_len = 2048 if M == 4 else 4096
+    max_len = {4: 2048, 8: 4096}.get(M, 8192)
     lens = np.round(np.exp(rng.uniform(np.log(1), np.log(max_len), 2000))).astype(np.int64)
     lens[:5] = [0, 1, 2, 0, max_len]
     off, a, b, c, d = ragged_case(rng, lens, np.float64)
@@ -353,14 +466,23 @@ def test_ragged_mixed_vs_oracle(M):
 
 def test_ragged_mixed_rejects():
     rng = np.random.default_rng(41)
-    off, a, b, c, d = ragged_case(rng, [10, 5000, 5], np.float64)
-    td = dev(d)
-    with pytest.raises(Exception):  # longer than 4096
-        api.ragged_batch_mixed(dev(a), dev(b), dev(c), td, dev(off), max_len=8192, sync=True)
-    off, a, b, c, d = ragged_case(rng, [10, 4000, 5], np.float64)
-    with pytest.raises(Exception):  # max_len promises less than a system needs
-        api.ragged_batch_mixed(dev(a), dev(b), dev(c), dev(d), dev(off), max_len=1000,
+    off, a, b, c, d = ragged_case(rng, [10, 9000, 5], np.float64)
+    with pytest.raises(Exception):  # longer than 8192
+        api.ragged_batch_mixed(dev(a), dev(b), dev(c), dev(d), dev(off), max_len=9000,
                                sync=True)
+    with pytest.raises(Exception):  # max_len within the limit, one system beyond it
+        api.ragged_batch_mixed(dev(a), dev(b), dev(c), dev(d), dev(off), max_len=8192,
+                               sync=True)
+    off, a, b, c, d = ragged_case(rng, [10, 5000, 5], np.float64)
+    with api.forced_ragged(8):  # all operands in registers: 4096 rows at most
+        with pytest.raises(Exception):
+            api.ragged_batch_mixed(dev(a), dev(b), dev(c), dev(d), dev(off), max_len=5000,
+                                   sync=True)
+    off, a, b, c, d = ragged_case(rng, [10, 4000, 5], np.float64)
+    with api.forced_ragged(4):  # 2048 rows at most; max_len promises less than needed
+        with pytest.raises(Exception):
+            api.ragged_batch_mixed(dev(a), dev(b), dev(c), dev(d), dev(off), max_len=1000,
+                                   sync=True)
     with pytest.raises(TypeError):
         api.ragged_batch_mixed(dev(a.astype(np.float32)), dev(b.astype(np.float32)),
                                dev(c.astype(np.float32)), dev(d.astype(np.float32)), dev(off),

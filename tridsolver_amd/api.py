@@ -234,7 +234,7 @@ def ragged_batch(a, b, c, d, offsets, *, max_len, u=None, sync=False, stream=Non
 def ragged_batch_mixed(a, b, c, d, offsets, *, max_len, refinements=2, sync=False,
                        stream=None):
     """FP64 ragged batch by FP32 sweeps plus FP64 refinement
-    (tridDmtsvRaggedBatchMixed); lengths up to 4096."""
+    (tridDmtsvRaggedBatchMixed); lengths up to 8192."""
     if d.dtype != torch.float64:
         raise TypeError("the mixed-precision ragged solve takes FP64 arrays")
     if offsets.dtype != torch.int64 or offsets.device != d.device:

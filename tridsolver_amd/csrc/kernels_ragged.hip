@@ -146,7 +146,8 @@ __global__ __launch_bounds__(RG_PLAN_BLOCK) void rg_scatter(const int64_t *__res
 struct RgMeta {
   int cls;
   bool active;
-  int64_t base, N;
+  int N;  // listed systems passed rg_class: N <= M * Tmax
+  int64_t base;
 };
 
 __device__ __forceinline__ RgMeta rg_meta(const RaggedPlan *plan, const int *__restrict__ list,
@@ -162,7 +163,7 @@ __device__ __forceinline__ RgMeta rg_meta(const RaggedPlan *plan, const int *__r
   m.active = slot < plan->sys_prefix[m.cls + 1];
   const int64_t sys = m.active ? list[slot] : 0;
   m.base = m.active ? off[sys] : 0;
-  m.N = m.active ? off[sys + 1] - off[sys] : 0;
+  m.N = m.active ? (int)(off[sys + 1] - off[sys]) : 0;
   return m;
 }
 
@@ -354,9 +355,96 @@ hipError_t launch_mixed_m(const double *a, const double *b, const double *c, dou
   return hipGetLastError();
 }
 
+// M = 16 with the FP64 a and c operands in LDS (mixed_contig_lds): 8192 rows
+// in 512 threads. Same split as the FP64 solve: systems needing more than
+// 256 lanes in one 512-thread workgroup per CU, the rest in 256-thread
+// workgroups, two per CU (LDS 2 tiles x T*(M+1) doubles + 2 ring slots of
+// 2 doubles per thread: 152 KiB at 512 threads, 76 KiB at 256).
+constexpr int RG_MIXED_LDS_M = 16;
+
+template <int T>
+constexpr size_t rg_mixed_lds_bytes() {
+  return 2 * sizeof(double) * (size_t)T * (RG_MIXED_LDS_M + 1) + hyb::ring_bytes<double>(T, 2);
+}
+
+// Two waves per SIMD in both shapes (the 256-thread workgroups run two per
+// CU), i.e. at most 256 VGPRs.
+template <int T, bool LONG>
+__global__ __launch_bounds__(T, 2) void rg_solve_mixed_lds(
+    const double *__restrict__ a, const double *__restrict__ b, const double *__restrict__ c,
+    double *__restrict__ d, const int64_t *__restrict__ off, const int *__restrict__ list,
+    RaggedPlan *plan, int refine) {
+  constexpr int M = RG_MIXED_LDS_M;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ int s_next;
+  const int t = threadIdx.x;
+  int *counter = LONG ? &plan->work : &plan->work_short;
+  const int lo = LONG ? 0 : plan->long_items;
+  const int total = LONG ? plan->long_items : plan->total_items;
+  if (t == 0) s_next = lo + atomicAdd(counter, 1);
+  __syncthreads();
+  int item = s_next;
+  RgMeta cur = item < total ? rg_meta(plan, list, off, item, t, T) : RgMeta{};
+  double *tiles = reinterpret_cast<double *>(smem_raw);
+  hyb::Ring ring{smem_raw + 2 * sizeof(double) * (size_t)T * (M + 1),
+                 (int)(2 * sizeof(double) * T), 0};
+  for (;;) {
+    __syncthreads();  // every thread has read s_next
+    if (item >= total) break;
+    // claim the following item now; its id is read after the first transpose
+    if (t == 0) s_next = lo + atomicAdd(counter, 1);
+    const hyb::Lane L = rg_lane(cur, t, T);
+    double *tA = tiles + (size_t)L.w * L.P * (M + 1);
+    double *tC = tA + (size_t)T * (M + 1);
+    int nxt = 0;
+    RgMeta next{};
+    // at most T lanes per system: log2(T) PCR steps
+    mixed_contig_lds<M, T == 512 ? 9 : 8>(L, ring, a, b, c, d, tA, tC, refine, [&] {
+      nxt = s_next;
+      if (nxt < total) next = rg_meta(plan, list, off, nxt, t, T);
+    });
+    item = nxt;
+    cur = next;
+  }
+}
+
+template <int T, bool LONG>
+hipError_t launch_rg_mixed_lds(const double *a, const double *b, const double *c, double *d,
+                               const int64_t *off, int *list, RaggedPlan *plan, unsigned blocks,
+                               int refine, hipStream_t st) {
+  auto kern = rg_solve_mixed_lds<T, LONG>;
+  static std::atomic<size_t> lds_granted{0};
+  hipError_t e = ensure_lds(reinterpret_cast<const void *>(kern), rg_mixed_lds_bytes<T>(),
+                            lds_granted);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(kern, dim3(blocks), dim3(T), rg_mixed_lds_bytes<T>(), st, a, b, c, d, off,
+                     list, plan, refine);
+  return hipGetLastError();
+}
+
+hipError_t launch_mixed_lds(const double *a, const double *b, const double *c, double *d,
+                            const int64_t *off, int64_t S, int per_cu, int refine, void *ws,
+                            hipStream_t st, int dev) {
+  constexpr int Tmax = 512, Tshort = 256;
+  RaggedPlan *plan = static_cast<RaggedPlan *>(ws);
+  int *list = reinterpret_cast<int *>(static_cast<unsigned char *>(ws) + 512);
+  launch_plan(off, S, RG_MIXED_LDS_M, Tmax, Tshort, ws, st);
+  int cus = 256;
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  const int per = std::max(1, per_cu);
+  hipError_t e = launch_rg_mixed_lds<Tmax, true>(a, b, c, d, off, list, plan,
+                                                 (unsigned)(cus * per), refine, st);
+  if (e != hipSuccess) return e;
+  return launch_rg_mixed_lds<Tshort, false>(a, b, c, d, off, list, plan,
+                                            (unsigned)(2 * cus * per), refine, st);
+}
+
 }  // namespace
 
-int ragged_mixed_max_len(int M) { return (M == 4 || M == 8) ? M * RG_MIXED_THREADS : 0; }
+int ragged_mixed_max_len(int M) {
+  if (M == RG_MIXED_LDS_M) return M * 512;
+  return (M == 4 || M == 8) ? M * RG_MIXED_THREADS : 0;
+}
 
 hipError_t launch_ragged_mixed(int M, const double *a, const double *b, const double *c,
                                double *d, const int64_t *off, int64_t S, int per_cu, int refine,
@@ -364,6 +452,7 @@ hipError_t launch_ragged_mixed(int M, const double *a, const double *b, const do
   switch (M) {
     case 4: return launch_mixed_m<4>(a, b, c, d, off, S, per_cu, refine, ws, st, dev);
     case 8: return launch_mixed_m<8>(a, b, c, d, off, S, per_cu, refine, ws, st, dev);
+    case RG_MIXED_LDS_M: return launch_mixed_lds(a, b, c, d, off, S, per_cu, refine, ws, st, dev);
     default: return hipErrorInvalidValue;
   }
 }

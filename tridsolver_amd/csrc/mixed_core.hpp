@@ -2,18 +2,24 @@
 // iterative refinement (device code shared by the strided and ragged
 // kernels). No counterpart in the reference.
 //
-// The lane holds the FP64 operands A, B, Cc, D of its M rows (chunk order).
-//   1. the matrix is factored in FP32: the spike sweeps' multipliers
-//      (1/beta_i, ga_i, ap_i, gp_i per row) and, for every PCR step, the
+// The lane holds the FP64 operands B, D of its M rows (chunk order) in
+// registers; A and Cc are read through accessors (registers, or an LDS tile
+// when the register file cannot hold all four operands).
+//   1. the matrix is factored in FP32: the forward spike sweep's multipliers
+//      (1/beta_i, ga_i, al_i per row), the chunk's first row in reduced form
+//      (backward recurrences, scalars only) and, for every PCR step, the
 //      pre-step coefficients (A, C, 1/(1 - A Cm - C Ap)) -- none of these
 //      depend on the right-hand side;
 //   2. x0 = FP32 solve of (float) d with that factorisation (right-hand-side
-//      recurrences only: de_i = (rhs_i - a_i de_{i-1}) / beta_i, backward
-//      dp_i = de_i - ga_i dp_{i+1}, PCR on D only, substitution);
+//      recurrences only: de_i = (rhs_i - a_i de_{i-1}) / beta_i, the first
+//      row's dp_0 by the backward recurrence dp_i = de_i - ga_i dp_{i+1}, PCR
+//      on D only, then x_i = de_i - al_i x_{s-1} - ga_i x_{i+1} from the
+//      chunk's end);
 //   3. `refine` times: r = d - A x in FP64 (neighbour chunks' boundary x via
 //      LDS), dx = FP32 solve of (float) r, x += dx in FP64.
 // The residual and the accumulation are the only FP64 arithmetic. P, the
-// lanes per system, is at most 2^MIXED_MAX_STEPS.
+// lanes per system, is at most 2^STEPS (STEPS <= MIXED_MAX_STEPS; a caller
+// whose workgroups have fewer lanes passes fewer steps and saves registers).
 
 #pragma once
 
@@ -23,33 +29,41 @@ namespace trid {
 
 constexpr int MIXED_MAX_STEPS = 9;  // PCR steps for P <= 512
 
-template <int M>
-__device__ __forceinline__ void mixed_chunk(const hyb::Lane &L, hyb::Ring &ring,
-                                            const double (&A)[M], const double (&B)[M],
-                                            const double (&Cc)[M], const double (&D)[M],
-                                            int refine, double (&X)[M]) {
+// Aof(r), Cof(r): the chunk's FP64 sub- and super-diagonal at row r (0 where
+// masked). The ring needs slots of 2 doubles per thread.
+template <int M, int STEPS, typename GetA, typename GetC>
+__device__ __forceinline__ void mixed_chunk_acc(const hyb::Lane &L, hyb::Ring &ring, GetA Aof,
+                                                const double (&B)[M], GetC Cof,
+                                                const double (&D)[M], int refine,
+                                                double (&X)[M]) {
   const int T = L.T, t = L.t, p = L.p, ps = L.pstep, P = L.P;
 
   // ---- FP32 factorisation ------------------------------------------------
-  float inv[M], ga[M], ap[M], gp[M];
+  // Per row: inv = 1/beta, ga and the forward spike al. The backward sweep
+  // is kept only for the chunk's first row (f_a, f_g); the substitution runs
+  // from the chunk's end on the forward values, so ap and gp are not stored.
+  float inv[M], ga[M], al[M];
   inv[0] = 1.0f / (float)B[0];
-  ap[0] = (float)A[0] * inv[0];  // forward spike al, becomes ap below
-  ga[0] = (float)Cc[0] * inv[0];
+  al[0] = (float)Aof(0) * inv[0];
+  ga[0] = (float)Cof(0) * inv[0];
 #pragma unroll
   for (int r = 1; r < M; ++r) {
-    const float ar = (float)A[r];
+    const float ar = (float)Aof(r);
     inv[r] = 1.0f / ((float)B[r] - ar * ga[r - 1]);
-    ap[r] = -ar * ap[r - 1] * inv[r];
-    ga[r] = (float)Cc[r] * inv[r];
+    al[r] = -ar * al[r - 1] * inv[r];
+    ga[r] = (float)Cof(r) * inv[r];
   }
-  const float al_e = ap[M - 1], ga_e = ga[M - 1];
-  if (M >= 2) gp[M - 2] = ga[M - 2];
+  const float al_e = al[M - 1], ga_e = ga[M - 1];
+  float f_a = 0.0f, f_g = -1.0f;
+  if (M >= 2) {
+    f_a = al[M - 2];
+    f_g = ga[M - 2];
 #pragma unroll
-  for (int r = M - 3; r >= 0; --r) {
-    ap[r] = ap[r] - ga[r] * ap[r + 1];
-    gp[r] = -ga[r] * gp[r + 1];
+    for (int r = M - 3; r >= 0; --r) {
+      f_a = al[r] - ga[r] * f_a;
+      f_g = -ga[r] * f_g;
+    }
   }
-  const float f_a = M >= 2 ? ap[0] : 0.0f, f_g = M >= 2 ? gp[0] : -1.0f;
   float na = 0, ng = 0;
   {
     float *q = ring.next<float>();
@@ -59,9 +73,9 @@ __device__ __forceinline__ void mixed_chunk(const hyb::Lane &L, hyb::Ring &ring,
   }
   const float invB = 1.0f / (1.0f - ga_e * na);
   float RA = al_e * invB, RC = -ga_e * ng * invB;
-  float sA[MIXED_MAX_STEPS], sC[MIXED_MAX_STEPS], sR[MIXED_MAX_STEPS];
+  float sA[STEPS], sC[STEPS], sR[STEPS];
 #pragma unroll
-  for (int s = 0; s < MIXED_MAX_STEPS; ++s) {
+  for (int s = 0; s < STEPS; ++s) {
     const int h = 1 << s;
     sA[s] = RA; sC[s] = RC; sR[s] = 1.0f;
     if (h < P) {
@@ -82,11 +96,14 @@ __device__ __forceinline__ void mixed_chunk(const hyb::Lane &L, hyb::Ring &ring,
   auto solve_rhs = [&](float (&v)[M]) {
     v[0] = v[0] * inv[0];
 #pragma unroll
-    for (int r = 1; r < M; ++r) v[r] = (v[r] - (float)A[r] * v[r - 1]) * inv[r];
+    for (int r = 1; r < M; ++r) v[r] = (v[r] - (float)Aof(r) * v[r - 1]) * inv[r];
     const float de_e = v[M - 1];
+    float f_d = 0.0f;
+    if (M >= 2) {
+      f_d = v[M - 2];
 #pragma unroll
-    for (int r = M - 3; r >= 0; --r) v[r] = v[r] - ga[r] * v[r + 1];
-    const float f_d = M >= 2 ? v[0] : 0.0f;
+      for (int r = M - 3; r >= 0; --r) f_d = v[r] - ga[r] * f_d;
+    }
     float nd = 0;
     {
       float *q = ring.next<float>();
@@ -96,7 +113,7 @@ __device__ __forceinline__ void mixed_chunk(const hyb::Lane &L, hyb::Ring &ring,
     }
     float Dr = (de_e - ga_e * nd) * invB;
 #pragma unroll
-    for (int s = 0; s < MIXED_MAX_STEPS; ++s) {
+    for (int s = 0; s < STEPS; ++s) {
       const int h = 1 << s;
       if (h < P) {
         float *q = ring.next<float>();
@@ -115,9 +132,9 @@ __device__ __forceinline__ void mixed_chunk(const hyb::Lane &L, hyb::Ring &ring,
       __syncthreads();
       if (p > 0) ym = q[t - ps];
     }
-#pragma unroll
-    for (int r = 0; r < M - 1; ++r) v[r] = v[r] - ap[r] * ym - gp[r] * y;
     v[M - 1] = y;
+#pragma unroll
+    for (int r = M - 2; r >= 0; --r) v[r] = v[r] - al[r] * ym - ga[r] * v[r + 1];
   };
 
   {
@@ -143,13 +160,82 @@ __device__ __forceinline__ void mixed_chunk(const hyb::Lane &L, hyb::Ring &ring,
     for (int r = 0; r < M; ++r) {
       const double xl = r > 0 ? X[r - 1] : xprev;
       const double xr = r < M - 1 ? X[r + 1] : xnext;
-      const double res = D[r] - (B[r] * X[r] + A[r] * xl + Cc[r] * xr);
+      const double res = D[r] - (B[r] * X[r] + Aof(r) * xl + Cof(r) * xr);
       v[r] = (float)res;
     }
     solve_rhs(v);
 #pragma unroll
     for (int r = 0; r < M; ++r) X[r] += (double)v[r];
   }
+}
+
+// All four operands in registers.
+template <int M>
+__device__ __forceinline__ void mixed_chunk(const hyb::Lane &L, hyb::Ring &ring,
+                                            const double (&A)[M], const double (&B)[M],
+                                            const double (&Cc)[M], const double (&D)[M],
+                                            int refine, double (&X)[M]) {
+  mixed_chunk_acc<M, MIXED_MAX_STEPS>(
+      L, ring, [&](int r) { return A[r]; }, B, [&](int r) { return Cc[r]; }, D, refine, X);
+}
+
+// Contiguous system (lane p reads rows p + P*r) with A and Cc kept in this
+// system's LDS tiles tA, tC of P*(M+1) doubles each: b and d are transposed
+// to chunk order through tA first, then a and c are written there skewed and
+// stay; lane p reads only its own chunk slots p*(M+1)+r of them, so X goes
+// back to fetch order through tA without a barrier before the writes. `ring`
+// needs slots of 2 doubles per thread. `fetched()` runs once the loads are
+// issued and a barrier has passed (a persistent caller decodes its next
+// item there).
+// Rows are 32-bit here (p + P*r < P*M <= 8192) and each array is addressed
+// from one per-lane pointer, so no 64-bit row index per row stays live (the
+// shared fetch's would, through to the store, and spill at this register
+// pressure).
+template <int M, int STEPS, typename OnFetched>
+__device__ __forceinline__ void mixed_contig_lds(const hyb::Lane &L, hyb::Ring &ring,
+                                                 const double *__restrict__ a,
+                                                 const double *__restrict__ b,
+                                                 const double *__restrict__ c,
+                                                 double *__restrict__ d, double *tA, double *tC,
+                                                 int refine, OnFetched fetched) {
+  const int P = L.P, p = L.p, N = L.active ? (int)L.N : 0;
+  const int64_t lb = L.base + p;
+  double A[M], B[M], Cc[M], D[M];
+  {
+    const double *pa = a + lb, *pb = b + lb, *pc = c + lb, *pd = d + lb;
+#pragma unroll
+    for (int r = 0; r < M; ++r) {
+      const int row = p + P * r;
+      const bool ok = row < N;
+      A[r] = (ok && row > 0) ? pa[P * r] : 0.0;
+      B[r] = ok ? pb[P * r] : 1.0;
+      Cc[r] = (row < N - 1) ? pc[P * r] : 0.0;
+      D[r] = ok ? pd[P * r] : 0.0;
+    }
+  }
+  hyb::to_chunk<M, true>(L, B, tA);
+  fetched();
+  hyb::to_chunk<M, true>(L, D, tA);
+#pragma unroll
+  for (int r = 0; r < M; ++r) {
+    const int e = hyb::skew<M>(L.p + L.P * r);
+    tA[e] = A[r];
+    tC[e] = Cc[r];
+  }
+  __syncthreads();
+  const double *ca = tA + L.p * (M + 1), *cc = tC + L.p * (M + 1);
+  double X[M];
+  mixed_chunk_acc<M, STEPS>(
+      L, ring, [&](int r) { return ca[r]; }, B, [&](int r) { return cc[r]; }, D, refine, X);
+  hyb::to_fetch<M, true>(L, X, tA);
+  // the store base is recomputed behind an empty asm, so that the load
+  // addresses are not kept live across the solve
+  int64_t sb = lb;
+  asm volatile("" : "+v"(sb));
+  double *qd = d + sb;
+#pragma unroll
+  for (int r = 0; r < M; ++r)
+    if (p + P * r < N) qd[P * r] = X[r];
 }
 
 }  // namespace trid

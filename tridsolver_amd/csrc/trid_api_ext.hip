@@ -54,9 +54,9 @@ tridStatus_t ragged_mixed(const TridParams *ctx, const double *a, const double *
   if (s != TRID_STATUS_SUCCESS) return s;
   hipStream_t st = current_stream();
   if (nsys == 0) return finish(ctx, st, hipSuccess);
-  // rows per lane: the forced value if it is a mixed one, else the smallest
-  // that covers max_len
-  int M = (tl_ragged_m == 4 || tl_ragged_m == 8) ? tl_ragged_m : (max_len <= 2048 ? 4 : 8);
+  // rows per lane: the forced value, else 16 (a and c in LDS, up to 8192
+  // rows; M = 4 and 8 keep every operand in registers, up to 2048 / 4096)
+  const int M = tl_ragged_m ? tl_ragged_m : 16;
   const int supported = ragged_mixed_max_len(M);
   if (supported == 0 || max_len > supported) return TRID_STATUS_INVALID_VALUE;
   void *ws = nullptr;

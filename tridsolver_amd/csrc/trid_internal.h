@@ -34,8 +34,10 @@ struct Geom {
 // run on the same XCD at the same time (workgroups are dispatched to the 8
 // XCDs round-robin; G = 1 keeps the plain order).
 // V selects the kernel variant: 0 the plain hybrid (all loads before the
-// sweep), 1 the streamed strided variant (loads a few rows ahead of the sweep,
-// for 256-byte row segments; kernels_stream.hip).
+// sweep); 1 and 2 the streamed strided variants (rows loaded a few ahead of
+// the sweep; kernels_stream.hip); 3 the strided variant with `a` staged in
+// LDS (kernels_wide.hip). For the mixed-precision solve, 3 is the M = 16
+// kernel with the FP64 a and c operands in LDS (kernels_mixed.hip).
 struct HybridCfg {
   int M = 0;
   int P = 0;

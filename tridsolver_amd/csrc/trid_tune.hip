@@ -289,7 +289,9 @@ tridStatus_t tune_mixed(int dev, const Geom &g, Choice *best) {
                      b, c, d, (double *)nullptr, n);
   best->algo = 0;
   best->ms = INFINITY;
-  static const int Ms[] = {4, 8};
+  // M = 4, 8: every operand in registers; M = 16: a and c in LDS (V = 3,
+  // P rounded up to whole waves for strided systems)
+  static const int Ms[] = {4, 8, 16};
   static const int Ws[] = {1, 2, 4, 8, 16, 32};
   static const int Gs[] = {1, 8, 16};
   for (int M : Ms)
@@ -300,6 +302,10 @@ tridStatus_t tune_mixed(int dev, const Geom &g, Choice *best) {
         cfg.P = (int)((g.N + M - 1) / M);
         cfg.W = W;
         cfg.G = G;
+        if (M == 16) {
+          cfg.V = 3;
+          if (g.stride != 1 && W < 64) cfg.P = (cfg.P + 64 / W - 1) / (64 / W) * (64 / W);
+        }
         if ((int64_t)W > g.n0 && W > 1) continue;
         if (!mixed_cfg_valid(g, cfg)) continue;
         const float ms = time_on(st, [&] { return launch_mixed(g, cfg, 2, a, b, c, d, st); });
