@@ -134,7 +134,7 @@ EXTERN_C tridStatus_t tridSmtsvStridedBatchAdjoint(const TridParams *ctx, const 
  * offsets[s] .. offsets[s+1]-1 of a, b, c, d (offsets: nsys+1 int64 values in
  * device memory, non-decreasing). Within a system the data contract of
  * tridsolver.h holds (a[first] and c[last] unused). max_len is an upper bound
- * of every length, known to the caller (at most 8192, or 4096 with FP32); a
+ * of every length, known to the caller (at most 8192); a
  * longer system found on the device is skipped and, for a synchronous call,
  * reported as TRID_STATUS_INVALID_VALUE. One persistent launch after four
  * small planning kernels; capturable. Needs tridHipRaggedWorkspaceBytes(nsys)
@@ -167,7 +167,7 @@ EXTERN_C tridStatus_t tridDmtsvRaggedBatchMixed(const TridParams *ctx, const dou
                                                 const int64_t *offsets, int64_t nsys,
                                                 int64_t max_len, int refinements);
 EXTERN_C size_t tridHipRaggedWorkspaceBytes(int64_t nsys);
-/* Forces the ragged kernel's rows per lane (4, 8, or 16 at FP64) and
+/* Forces the ragged kernel's rows per lane (4, 8 or 16) and
  * workgroups per CU for this thread's calls; M = 0 restores the tuned choice. */
 EXTERN_C tridStatus_t tridHipForceRaggedConfig(int M, int per_cu);
 

@@ -332,8 +332,6 @@ def check_ragged(off, a, b, c, d, got, dtype, u0=None):
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("M", [0, 4, 8, 16])
 def test_ragged_vs_oracle(dtype, M):
-    if M == 16 and dtype == np.float32:
-        pytest.skip("FP32 ragged kernel is built for M = 4, 8")
     rng = np.random.default_rng(M + 1)
     max_len = 2048 if M == 4 and dtype == np.float64 else 4096
     lens = np.round(np.exp(rng.uniform(np.log(1), np.log(max_len), 3000))).astype(np.int64)

@@ -185,11 +185,12 @@ __device__ __forceinline__ hyb::Lane rg_lane(const RgMeta &m, int t, int Tmax) {
   return L;
 }
 
-// Threads per workgroup: the register file holds 1024 threads of the M = 4/8
-// variants and 512 of FP64 M = 16 (FP32 M = 16 is not built: it spills at 1024).
+// Threads per workgroup: 1024 for the M = 4/8 variants, 512 for M = 16 (FP64
+// M = 16 needs that register budget; FP32 M = 16 at 1024 threads would spill,
+// and 512 threads of 16 rows cover the same 8192 rows with half the lanes).
 template <typename REAL, int M>
 struct RgBounds {
-  static constexpr int threads = sizeof(REAL) * M > 64 ? 512 : 1024;
+  static constexpr int threads = M >= 16 ? 512 : 1024;
 };
 
 // One persistent launch over a range of items: LONG = the items whose systems
@@ -460,9 +461,9 @@ hipError_t launch_ragged_mixed(int M, const double *a, const double *b, const do
 size_t ragged_workspace_bytes(int64_t S) { return 512 + sizeof(int) * (size_t)std::max<int64_t>(S, 1); }
 
 int ragged_max_len(int fp64, int M) {
-  if (M != 4 && M != 8 && !(M == 16 && fp64)) return 0;
-  const int T = (fp64 ? 8 : 4) * M > 64 ? 512 : 1024;
-  return M * T;
+  (void)fp64;  // the same shapes at both precisions
+  if (M != 4 && M != 8 && M != 16) return 0;
+  return M * (M >= 16 ? 512 : 1024);
 }
 
 template <typename REAL, bool INC>
@@ -473,10 +474,7 @@ hipError_t launch_ragged(int M, const REAL *a, const REAL *b, const REAL *c, REA
   switch (M) {
     case 4: return launch_m<REAL, 4, INC>(a, b, c, d, u, off, S, per_cu, ws, st, dev);
     case 8: return launch_m<REAL, 8, INC>(a, b, c, d, u, off, S, per_cu, ws, st, dev);
-    case 16:
-      if constexpr (sizeof(REAL) == 8)
-        return launch_m<REAL, 16, INC>(a, b, c, d, u, off, S, per_cu, ws, st, dev);
-      return hipErrorInvalidValue;
+    case 16: return launch_m<REAL, 16, INC>(a, b, c, d, u, off, S, per_cu, ws, st, dev);
     default: return hipErrorInvalidValue;
   }
 }
