@@ -161,6 +161,79 @@ def cpu_baseline(n, slab, reps=3):
     return out
 
 
+def cpu_extras(reps=3):
+    """CPU timings of the other workloads on bounded samples, OpenMP on all
+    threads: ragged (the first 4,000 systems of the 40,000-system set) and the
+    FP64 adjoint (all gradients, 256x256x64) by the oracle restatement; one
+    FP64 and one FP32 ADI step at 256^3 by the reference CPU solver where it
+    is built. Median of `reps` after a warm-up."""
+    import numpy as np
+
+    import oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+
+    def med(fn):
+        fn()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return statistics.median(ts)
+
+    out = {"cores": threads}
+    rng = np.random.default_rng(40000)
+    lens = np.round(np.exp(rng.uniform(np.log(8), np.log(8192), 40000))).astype(np.int64)[:4000]
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    npts = int(off[-1])
+    a = rng.uniform(-1, 1, npts)
+    c = rng.uniform(-1, 1, npts)
+    b = 2 + np.abs(a) + np.abs(c) + rng.uniform(0, 1, npts)
+    d = rng.uniform(-1, 1, npts)
+    x = d.copy()
+    t = med(lambda: oracle.ragged(a, b, c, x, off, nthreads=threads))
+    out["ragged_fp64"] = {"gpts_s": round(npts / t / 1e9, 3), "kind": "port",
+                          "sample": "%d systems, %d unknowns, FP64" % (len(lens), npts)}
+    # ADI: the reference app's CPU step (apps/adi/src/adi_cpu.cpp:223-269):
+    # preproc, then the reference solver along x, y and z (Inc into u)
+    n = 256
+    dims3 = [n, n, n]
+    use_ref = oracle.ref_available()
+    for dt, key in ((np.float64, "adi_fp64_step"), (np.float32, "adi_fp32_step")):
+        u = rng.uniform(0, 1, n ** 3).astype(dt)
+        bufs = tuple(np.empty_like(u) for _ in range(4)) if use_ref else None
+
+        def step():
+            if not use_ref:
+                oracle.adi(1.0, u, n, n, n, 1, nthreads=threads)
+                return
+            du, a3, b3, c3 = oracle.adi_preproc(1.0, u, n, n, n, out=bufs)
+            oracle.ref_batch(a3, b3, c3, du, ndim=3, solvedim=0, dims=dims3)
+            oracle.ref_batch(a3, b3, c3, du, ndim=3, solvedim=1, dims=dims3)
+            oracle.ref_batch(a3, b3, c3, du, u, ndim=3, solvedim=2, dims=dims3, inc=True)
+
+        t = med(step)
+        del bufs
+        out[key] = {"ms": round(t * 1e3, 2), "gpts_s": round(n ** 3 / t / 1e9, 3),
+                    "kind": "reference" if use_ref else "port",
+                    "sample": "one ADI step at 256^3: preproc (oracle), x, y, z-Inc solves (%s)"
+                              % ("reference CPU library" if use_ref else "oracle")}
+    dims = [256, 256, 64]
+    m = int(np.prod(dims))
+    a = rng.uniform(-1, 1, m)
+    c = rng.uniform(-1, 1, m)
+    b = 2 + np.abs(a) + np.abs(c) + rng.uniform(0, 1, m)
+    xs = rng.uniform(-1, 1, m)
+    w = rng.standard_normal(m)
+    per = {}
+    for sd, name in enumerate("xyz"):
+        t = med(lambda: oracle.adjoint(a, b, c, xs, w, ndim=3, solvedim=sd, dims=dims,
+                                       nthreads=threads))
+        per[name] = round(m / t / 1e9, 3)
+    out["adjoint_fp64_all_grads_gpts_s"] = dict(per, kind="port", sample="256x256x64 FP64")
+    return out
+
+
 def bench_fp32_dirs(torch, api, n, tune):
     n3 = n ** 3
     a, b, c, d = gen_gpu(torch, n3, torch.float32, seed=32)
@@ -442,6 +515,9 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(n, args.cpu_slab)
         log("cpu: %s" % cpu)
+        if not args.quick:
+            extras["cpu_other_workloads"] = cpu_extras()
+            log("cpu extras: %s" % extras["cpu_other_workloads"])
 
     out = {
         "metric": "grid pts solved/s (FP64 512^3 x/y/z batched tridiagonal solves)",

@@ -157,9 +157,10 @@ def adjoint(a, b, c, x, w, *, ndim, solvedim, dims, pads=None, nthreads=0):
     return lam, ga, gb, gc
 
 
-def adi_preproc(lam, u, nx, ny, nz):
+def adi_preproc(lam, u, nx, ny, nz, out=None):
+    """(du, a, b, c) of the ADI preproc; `out` may supply the four arrays."""
     dt = u.dtype
-    du, a, b, c = (np.empty_like(u) for _ in range(4))
+    du, a, b, c = out if out is not None else (np.empty_like(u) for _ in range(4))
     f = lib().orc_adi_preproc_d if dt == np.float64 else lib().orc_adi_preproc_s
     f(lam, _ptr(u), _ptr(du), _ptr(a), _ptr(b), _ptr(c), nx, ny, nz)
     return du, a, b, c
