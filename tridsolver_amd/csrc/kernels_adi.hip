@@ -257,6 +257,54 @@ bool adi_cfg(int dev, const Geom &g, HybridCfg *cfg) {
   return hybrid_cfg_valid<REAL>(g, *cfg) || hybrid_plain_cfg<REAL>(g, cfg);
 }
 
+// Shape of the ADI x sweeps, shared by the array step's x solve and the
+// matrix-free x sweep so that the two steps agree bit for bit. An x line
+// fits one workgroup easily; what costs is the PCR across its lanes (a
+// division per step and lane), so fewer lanes of more rows than the general
+// contiguous default (up to 64 lanes of 4 rows at 256 rows) pay off. Measured
+// on 256^3 steps over M in {4, 8, 16}, T in {64, 128, 256}, G in {1, 8}:
+// 8 rows per lane, one 64-thread wave per workgroup, plain workgroup order
+// (matrix-free step FP64 0.217 against 0.229-0.233 ms at M = 4; FP32 0.149
+// against 0.159-0.162).
+template <typename REAL>
+struct AdiX {
+  static constexpr int M = 8;
+  static constexpr int T = 64;
+  static constexpr int G = 1;
+};
+
+template <typename REAL>
+bool adi_x_cfg(const Geom &g, HybridCfg *cfg) {
+  HybridCfg c;
+  c.M = AdiX<REAL>::M;
+  c.P = (int)((g.N + c.M - 1) / c.M);
+  c.W = (int)std::max<int64_t>(1, AdiX<REAL>::T / c.P);
+  c.G = AdiX<REAL>::G;
+  c.V = 0;
+  if (g.stride != 1 || !hybrid_cfg_valid<REAL>(g, c)) return false;
+  *cfg = c;
+  return true;
+}
+
+// The x solve of the array step, in the ADI x shape unless the caller has
+// forced a shape of its own.
+template <typename REAL>
+tridStatus_t x_solve(const TridParams *p, const REAL *a, const REAL *b, const REAL *c, REAL *d,
+                     const int *dims) {
+  Geom g;
+  bool empty = false;
+  HybridCfg xc;
+  const bool shape = forced_config().M == 0 &&
+                     make_geom(3, 0, dims, dims, &g, &empty) == TRID_STATUS_SUCCESS &&
+                     !empty && adi_x_cfg<REAL>(g, &xc);
+  if (shape) tridHipForceHybridConfig(xc.M, xc.P, xc.W, xc.G, xc.V);
+  tridStatus_t s;
+  if constexpr (sizeof(REAL) == 8) s = tridDmtsvStridedBatch(p, a, b, c, d, 3, 0, dims, dims);
+  else s = tridSmtsvStridedBatch(p, a, b, c, d, 3, 0, dims, dims);
+  if (shape) tridHipForceHybridConfig(0, 0, 0, 0, 0);
+  return s;
+}
+
 template <typename REAL, bool INC, bool RHS = false>
 tridStatus_t adi_sweep(int dev, REAL lambda, REAL *d, REAL *u, int nx, int ny, int nz,
                        int sd) {
@@ -267,7 +315,8 @@ tridStatus_t adi_sweep(int dev, REAL lambda, REAL *d, REAL *u, int nx, int ny, i
   tridStatus_t s = make_geom(3, sd, dims, dims, &g, &empty);
   if (s != TRID_STATUS_SUCCESS || empty) return s;
   HybridCfg cfg;
-  if (!adi_cfg<REAL, INC>(dev, g, &cfg)) return TRID_STATUS_INVALID_VALUE;
+  if (!(sd == 0 && adi_x_cfg<REAL>(g, &cfg)) && !adi_cfg<REAL, INC>(dev, g, &cfg))
+    return TRID_STATUS_INVALID_VALUE;
   AdiCoef<REAL> kc{lambda, nx, ny, nz, sd};
   hipStream_t st = current_stream();
   const bool contig = sd == 0;
@@ -333,7 +382,7 @@ tridStatus_t step<double>(const TridParams *ctx, double lambda, double *u, doubl
   const int dims[3] = {nx, ny, nz};
   tridStatus_t s = preproc<double>(lambda, u, du, a, b, c, nullptr, nullptr, nullptr,
                                    nullptr, nullptr, nullptr, nx, ny, nz);
-  if (s == TRID_STATUS_SUCCESS) s = tridDmtsvStridedBatch(&p, a, b, c, du, 3, 0, dims, dims);
+  if (s == TRID_STATUS_SUCCESS) s = x_solve<double>(&p, a, b, c, du, dims);
   if (s == TRID_STATUS_SUCCESS) s = tridDmtsvStridedBatch(&p, a, b, c, du, 3, 1, dims, dims);
   if (s == TRID_STATUS_SUCCESS)
     s = tridDmtsvStridedBatchInc(&p, a, b, c, du, u, 3, 2, dims, dims);
@@ -348,7 +397,7 @@ tridStatus_t step<float>(const TridParams *ctx, float lambda, float *u, float *d
   const int dims[3] = {nx, ny, nz};
   tridStatus_t s = preproc<float>(lambda, u, du, a, b, c, nullptr, nullptr, nullptr,
                                   nullptr, nullptr, nullptr, nx, ny, nz);
-  if (s == TRID_STATUS_SUCCESS) s = tridSmtsvStridedBatch(&p, a, b, c, du, 3, 0, dims, dims);
+  if (s == TRID_STATUS_SUCCESS) s = x_solve<float>(&p, a, b, c, du, dims);
   if (s == TRID_STATUS_SUCCESS) s = tridSmtsvStridedBatch(&p, a, b, c, du, 3, 1, dims, dims);
   if (s == TRID_STATUS_SUCCESS)
     s = tridSmtsvStridedBatchInc(&p, a, b, c, du, u, 3, 2, dims, dims);
