@@ -46,6 +46,10 @@ def parse():
     ap.add_argument("--cpu-slab", type=int, default=64,
                     help="z-extent of the CPU baseline sample (n x n x slab)")
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--only", choices=["fp32", "mixed", "adjoint", "ragged", "adi", "partitioned",
+                                       "cpu-extras"],
+                    help="run one secondary workload and print its JSON instead of the bench "
+                         "line (for profiling)")
     return ap.parse_args()
 
 
@@ -435,6 +439,31 @@ def main():
     torch.cuda.set_device(0)
     lib = _lib.lib()
     n = args.n
+    if args.only:
+        api.use_torch_stream()
+        tune = not args.no_tune
+        if args.only == "mixed":
+            a, b, c, d0 = gen_gpu(torch, n ** 3, torch.float64, seed=64)
+            ref = {}
+            for sd, name in enumerate("xyz"):
+                if tune:
+                    api.autotune(torch.float64, sd, [n] * 3)
+                d = d0.clone()
+                ref[name] = event_ms(torch, lambda: api.trid_strided_batch(
+                    a, b, c, d, solvedim=sd, dims=[n] * 3))
+            del a, b, c, d0, d
+            torch.cuda.empty_cache()
+            res = bench_mixed(torch, api, n, ref, tune)
+        else:
+            run = {"fp32": lambda: bench_fp32_dirs(torch, api, n, tune),
+                   "adjoint": lambda: bench_adjoint(torch, api),
+                   "ragged": lambda: bench_ragged(torch, api, tune),
+                   "adi": lambda: bench_adi(torch),
+                   "partitioned": lambda: bench_partitioned(torch, api),
+                   "cpu-extras": cpu_extras}[args.only]
+            res = run()
+        print(json.dumps({"only": args.only, "result": res}))
+        return
     n3 = n ** 3
     dims = [n, n, n]
     bytes_per_pt = 40.0

@@ -263,14 +263,16 @@ bool adi_cfg(int dev, const Geom &g, HybridCfg *cfg) {
 // division per step and lane), so fewer lanes of more rows than the general
 // contiguous default (up to 64 lanes of 4 rows at 256 rows) pay off. Measured
 // on 256^3 steps over M in {4, 8, 16}, T in {64, 128, 256}, G in {1, 8}:
-// 8 rows per lane, one 64-thread wave per workgroup, plain workgroup order
-// (matrix-free step FP64 0.217 against 0.229-0.233 ms at M = 4; FP32 0.149
-// against 0.159-0.162).
+// 8 rows per lane, one 64-thread wave per workgroup (matrix-free step FP64
+// 0.217 against 0.229-0.233 ms at M = 4; FP32 0.149 against 0.159-0.162).
+// G = 1 and 8 time alike; with G = 8 neighbouring x lines run on one XCD,
+// so the halo lines the right-hand side reads (j +- 1) hit in its L2
+// instead of being fetched again.
 template <typename REAL>
 struct AdiX {
   static constexpr int M = 8;
   static constexpr int T = 64;
-  static constexpr int G = 1;
+  static constexpr int G = 8;
 };
 
 template <typename REAL>
