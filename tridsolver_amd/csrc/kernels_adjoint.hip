@@ -106,7 +106,7 @@ hipError_t launch_adjoint_m(const Geom &g, const HybridCfg &cfg, const REAL *a,
   size_t lds = hyb::ring_bytes<REAL>(T, 3);
   if (CONTIG) lds += sizeof(REAL) * (size_t)T * (M + 1);
   auto kern = adjoint_kernel<REAL, M, CONTIG>;
-  static std::atomic<size_t> lds_granted{0};
+  static LdsGrant lds_granted;
   {
     hipError_t e = ensure_lds(reinterpret_cast<const void *>(kern), lds, lds_granted);
     if (e != hipSuccess) return e;

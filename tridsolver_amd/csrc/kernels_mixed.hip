@@ -61,7 +61,7 @@ hipError_t launch_mixed_m(const Geom &g, const HybridCfg &cfg, int refine, const
   size_t lds = hyb::ring_bytes<double>(T, 3);
   if (CONTIG) lds += sizeof(double) * (size_t)T * (M + 1);
   auto kern = mixed_kernel<M, CONTIG>;
-  static std::atomic<size_t> lds_granted{0};
+  static LdsGrant lds_granted;
   {
     hipError_t e = ensure_lds(reinterpret_cast<const void *>(kern), lds, lds_granted);
     if (e != hipSuccess) return e;
@@ -209,27 +209,27 @@ hipError_t launch_mixed_lds(const Geom &g, const HybridCfg &cfg, int refine, con
   const int T = cfg.W * cfg.P;
   const size_t lds = mixed_lds_bytes(g, cfg.P, cfg.W);
   const int64_t blocks = (g.n0 + cfg.W - 1) / cfg.W * g.n1;
-  auto go = [&](const void *kern, std::atomic<size_t> &granted, auto launch) {
+  auto go = [&](const void *kern, LdsGrant &granted, auto launch) {
     hipError_t e = ensure_lds(kern, lds, granted);
     if (e != hipSuccess) return e;
     launch();
     return hipGetLastError();
   };
   if (g.stride == 1) {
-    static std::atomic<size_t> granted{0};
+    static LdsGrant granted;
     return go(reinterpret_cast<const void *>(mixed_tile_kernel), granted, [&] {
       hipLaunchKernelGGL(mixed_tile_kernel, dim3((unsigned)blocks), dim3(T), lds, st, g, cfg.P,
                          cfg.W, cfg.G, refine, a, b, c, d);
     });
   }
   if (cfg.W * (int)sizeof(double) == 128) {
-    static std::atomic<size_t> granted{0};
+    static LdsGrant granted;
     return go(reinterpret_cast<const void *>(mixed_wide_kernel<128>), granted, [&] {
       hipLaunchKernelGGL(mixed_wide_kernel<128>, dim3((unsigned)blocks), dim3(T), lds, st, g,
                          cfg.P, cfg.G, refine, a, b, c, d);
     });
   }
-  static std::atomic<size_t> granted{0};
+  static LdsGrant granted;
   return go(reinterpret_cast<const void *>(mixed_wide_kernel<64>), granted, [&] {
     hipLaunchKernelGGL(mixed_wide_kernel<64>, dim3((unsigned)blocks), dim3(T), lds, st, g,
                        cfg.P, cfg.G, refine, a, b, c, d);

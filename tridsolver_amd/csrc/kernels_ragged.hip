@@ -273,7 +273,7 @@ hipError_t launch_m(const REAL *a, const REAL *b, const REAL *c, REAL *d, REAL *
   {  // the long systems first (an empty range exits at once)
     const size_t lds = sizeof(REAL) * (size_t)Tmax * (M + 1) + hyb::ring_bytes<REAL>(Tmax, 3);
     auto kern = rg_solve<REAL, M, INC, Tmax, true>;
-    static std::atomic<size_t> lds_granted{0};
+    static LdsGrant lds_granted;
     hipError_t e = ensure_lds(reinterpret_cast<const void *>(kern), lds, lds_granted);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(kern, dim3((unsigned)(cus * per)), dim3(Tmax), lds, st, a, b, c, d, u,
@@ -282,7 +282,7 @@ hipError_t launch_m(const REAL *a, const REAL *b, const REAL *c, REAL *d, REAL *
   }
   const size_t lds = sizeof(REAL) * (size_t)Tshort * (M + 1) + hyb::ring_bytes<REAL>(Tshort, 3);
   auto kern = rg_solve<REAL, M, INC, Tshort, false>;
-  static std::atomic<size_t> lds_granted{0};
+  static LdsGrant lds_granted;
   hipError_t e = ensure_lds(reinterpret_cast<const void *>(kern), lds, lds_granted);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(kern, dim3((unsigned)(2 * cus * per)), dim3(Tshort), lds, st, a, b, c, d,
@@ -344,7 +344,7 @@ hipError_t launch_mixed_m(const double *a, const double *b, const double *c, dou
   const size_t lds =
       sizeof(double) * (size_t)Tmax * (M + 1) + hyb::ring_bytes<double>(Tmax, 3);
   auto kern = rg_solve_mixed<M>;
-  static std::atomic<size_t> lds_granted{0};
+  static LdsGrant lds_granted;
   {
     hipError_t e = ensure_lds(reinterpret_cast<const void *>(kern), lds, lds_granted);
     if (e != hipSuccess) return e;
@@ -414,7 +414,7 @@ hipError_t launch_rg_mixed_lds(const double *a, const double *b, const double *c
                                const int64_t *off, int *list, RaggedPlan *plan, unsigned blocks,
                                int refine, hipStream_t st) {
   auto kern = rg_solve_mixed_lds<T, LONG>;
-  static std::atomic<size_t> lds_granted{0};
+  static LdsGrant lds_granted;
   hipError_t e = ensure_lds(reinterpret_cast<const void *>(kern), rg_mixed_lds_bytes<T>(),
                             lds_granted);
   if (e != hipSuccess) return e;

@@ -199,7 +199,7 @@ hipError_t launch_hybrid_m(const Geom &g, const HybridCfg &cfg, const REAL *a,
   size_t lds = sizeof(REAL) * (size_t)6 * T;
   if (CONTIG) lds += sizeof(REAL) * (size_t)cfg.W * cfg.P * (M + 1);
   auto kern = hybrid_kernel<REAL, M, INC, CONTIG, VEC>;
-  static std::atomic<size_t> lds_granted{0};
+  static LdsGrant lds_granted;
   {
     hipError_t e = ensure_lds(reinterpret_cast<const void *>(kern), lds, lds_granted);
     if (e != hipSuccess) return e;

@@ -196,7 +196,7 @@ hipError_t launch_lds_m(const Geom &g, const HybridCfg &cfg, const REAL *a, cons
   const int T = cfg.W * cfg.P;
   const size_t lds = sizeof(REAL) * ((size_t)M * T + (size_t)3 * T);
   auto kern = hybrid_lds_kernel<REAL, M, LA, INC>;
-  static std::atomic<size_t> lds_granted{0};
+  static LdsGrant lds_granted;
   {
     hipError_t e = ensure_lds(reinterpret_cast<const void *>(kern), lds, lds_granted);
     if (e != hipSuccess) return e;
@@ -214,7 +214,7 @@ hipError_t launch_stream_m(const Geom &g, const HybridCfg &cfg, const REAL *a, c
   const int T = cfg.W * cfg.P;
   const size_t lds = hyb::ring_bytes<REAL>(T, 3);
   auto kern = hybrid_stream_kernel<REAL, M, LA, INC>;
-  static std::atomic<size_t> lds_granted{0};
+  static LdsGrant lds_granted;
   {
     hipError_t e = ensure_lds(reinterpret_cast<const void *>(kern), lds, lds_granted);
     if (e != hipSuccess) return e;

@@ -212,7 +212,7 @@ hipError_t launch_wide_m(const Geom &g, const HybridCfg &cfg, const REAL *a, con
   const int slots = wide_slots<REAL>(M, cfg.P, SEG);
   const size_t lds = wide_lds_bytes<REAL>(M, cfg.P, SEG, slots);
   auto kern = hybrid_wide_kernel<REAL, M, SEG, INC>;
-  static std::atomic<size_t> lds_granted{0};
+  static LdsGrant lds_granted;
   {
     hipError_t e = ensure_lds(reinterpret_cast<const void *>(kern), lds, lds_granted);
     if (e != hipSuccess) return e;

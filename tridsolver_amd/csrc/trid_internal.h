@@ -117,13 +117,26 @@ hipError_t launch_ragged(int M, const REAL *a, const REAL *b, const REAL *c, REA
                          int dev);
 hipError_t ragged_error_flag(void *ws, int *flag);
 
-// Raises a kernel's dynamic-LDS limit to `lds` bytes when it exceeds the
-// default 64 KiB and what was granted before (not capturable: the first call
-// with a given size must happen outside hipGraph capture).
-inline hipError_t ensure_lds(const void *kern, size_t lds, std::atomic<size_t> &granted) {
-  if (lds <= 65536 || lds <= granted.load()) return hipSuccess;
-  hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (e == hipSuccess) granted.store(lds);
+// The dynamic-LDS limit granted to one kernel, per device (the attribute is
+// set on the current device; a process may drive several).
+constexpr int LDS_GRANT_DEVICES = 64;
+struct LdsGrant {
+  std::atomic<size_t> bytes[LDS_GRANT_DEVICES] = {};
+};
+
+// Raises a kernel's dynamic-LDS limit on the current device to `lds` bytes
+// when it exceeds the default 64 KiB and what was granted there before (not
+// capturable: the first call with a given size must happen outside hipGraph
+// capture).
+inline hipError_t ensure_lds(const void *kern, size_t lds, LdsGrant &granted) {
+  if (lds <= 65536) return hipSuccess;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= LDS_GRANT_DEVICES) return hipErrorInvalidDevice;
+  if (lds <= granted.bytes[dev].load()) return hipSuccess;
+  e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e == hipSuccess) granted.bytes[dev].store(lds);
   return e;
 }
 
