@@ -60,5 +60,25 @@ def build(verbose: bool = False) -> str:
     return out
 
 
+def build_c_caller() -> str:
+    """Compiles tests/c/drop_in.c, a plain C99 caller of the library through the
+    reference's header (gcc, linked against libtridhip.so and the HIP runtime,
+    found at run time through rpath). Returns the binary's path."""
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "c", "drop_in.c")
+    out = os.path.join(root, "tests", "c", "drop_in")
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    cmd = [os.environ.get("CC", "gcc"), "-std=c99", "-O2", "-Wall",
+           "-I" + os.path.join(root, "include"), "-I" + os.path.join(rocm, "include"),
+           "-D__HIP_PLATFORM_AMD__", src, "-o", out, "-L" + LIBDIR, "-ltridhip",
+           "-L" + os.path.join(rocm, "lib"), "-lamdhip64", "-lm",
+           "-Wl,-rpath,$ORIGIN/../../tridsolver_amd/lib", "-Wl,-rpath," + os.path.join(rocm, "lib")]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("C caller build failed:\n" + r.stdout + r.stderr)
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_c_caller())
