@@ -28,6 +28,10 @@ import statistics
 import sys
 import time
 
+# OpenMP threads of the CPU baseline stay on neighbouring cores (read by
+# libgomp when the CPU libraries load, so set before they are)
+os.environ.setdefault("OMP_PROC_BIND", "close")
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -94,7 +98,7 @@ def measure_copy_bw(torch, lib, nbytes=4 << 30, reps=10):
     return 2 * nbytes / (ms * 1e-3) / 1e9
 
 
-def cpu_baseline(n, slab, reps=3):
+def cpu_baseline(n, slab, reps=5):
     """Reference CPU solver (oracle/_ref, compiled from the reference sources)
     on an n x n x slab FP64 sample, x, y and z solves; falls back to the
     oracle restatement if the reference library is not present."""
@@ -150,10 +154,11 @@ def cpu_baseline(n, slab, reps=3):
     except OSError:
         pass
     desc = ("%s CPU solver, FP64 %dx%dx%d sample (x, y, z solves, median of %d), "
-            "%d OpenMP threads, %s, %s" % ("reference" if kind == "reference" else "oracle",
-                                          n, n, slab, reps, threads,
-                                          oracle.ref_isa() if kind == "reference" else "scalar",
-                                          model or "cpu model unknown"))
+            "%d OpenMP threads (OMP_PROC_BIND=%s), %s, %s" % (
+                "reference" if kind == "reference" else "oracle", n, n, slab, reps, threads,
+                os.environ.get("OMP_PROC_BIND", ""),
+                oracle.ref_isa() if kind == "reference" else "scalar",
+                model or "cpu model unknown"))
     out = {"value": pts_s, "unit": "grid pts/s", "cores": threads, "kind": kind,
            "sample": desc, "alg_GBs": round(40.0 * pts_s / 1e9, 1),
            "per_dir_gpts_s": {k: round(npts / v / 1e9, 3) for k, v in per_dir.items()},
