@@ -19,7 +19,8 @@
  *   - invalid arguments return TRID_STATUS_INVALID_VALUE instead of being
  *     undefined behaviour; HIP failures return TRID_STATUS_EXECUTION_FAILED
  *     instead of exit(-1);
- *   - there is no N_MAX=1024 length limit;
+ *   - there is no N_MAX=1024 length limit; a call solves at most 2^31 - 1
+ *     systems (the product of the non-solve dimensions);
  *   - ctx may be NULL; ctx->sync == 0 makes the call asynchronous on the
  *     library stream (see tridsolver_hip.h);
  *   - Inc never writes d on any algorithm (the reference's x options 2/3

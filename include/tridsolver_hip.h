@@ -132,7 +132,7 @@ EXTERN_C tridStatus_t tridSmtsvStridedBatchAdjoint(const TridParams *ctx, const 
  * No counterpart in the reference (its API needs one length for all systems).
  * nsys independent systems stored back to back: system s occupies elements
  * offsets[s] .. offsets[s+1]-1 of a, b, c, d (offsets: nsys+1 int64 values in
- * device memory, non-decreasing). Within a system the data contract of
+ * device memory, non-decreasing; nsys <= 2^31 - 1). Within a system the data contract of
  * tridsolver.h holds (a[first] and c[last] unused). max_len is an upper bound
  * of every length, known to the caller (at most 8192); a
  * longer system found on the device is skipped and, for a synchronous call,

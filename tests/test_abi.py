@@ -71,6 +71,8 @@ def _call_invalid(**kw):
     dict(ndim=0), dict(ndim=4), dict(solvedim=3), dict(solvedim=-1),
     dict(dims=[4, -1, 4]), dict(pads=[3, 4, 4]), dict(dims=None), dict(pads=None),
     dict(ptr=None),
+    # 2^32 systems: more than one call solves (32-bit grids and system indices)
+    dict(dims=[2, 65536, 65536], pads=[2, 65536, 65536]),
 ])
 def test_invalid_arguments_rejected_without_device(kw):
     assert _call_invalid(**kw) == 3  # TRID_STATUS_INVALID_VALUE

@@ -12,6 +12,7 @@
 // kernel, opts are hints, and ctx->sync is honoured.
 
 #include <algorithm>
+#include <cstdint>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -114,6 +115,12 @@ tridStatus_t make_geom(int ndim, int solvedim, const int *dims, const int *pads,
     dm[i] = dims[i];
     pd[i] = pads[i];
   }
+  // launch grids have at most one workgroup per system and HIP grids are
+  // 32-bit: at most 2^31 - 1 systems per call
+  int64_t systems = 1;
+  for (int k = 0; k < 3; ++k)
+    if (k != solvedim) systems *= dm[k];
+  if (systems > INT32_MAX) return TRID_STATUS_INVALID_VALUE;
   g->N = dm[solvedim];
   if (solvedim == 0) {
     g->n0 = dm[1]; g->n1 = dm[2]; g->st0 = pd[0]; g->st1 = pd[0] * pd[1]; g->stride = 1;

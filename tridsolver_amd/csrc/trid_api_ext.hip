@@ -3,6 +3,7 @@
 // batches. None has a counterpart in the reference (see include/tridsolver_hip.h).
 
 #include <algorithm>
+#include <cstdint>
 #include <mutex>
 
 #include "../../include/tridsolver_hip.h"
@@ -17,8 +18,9 @@ thread_local int tl_ragged_m = 0, tl_ragged_per_cu = 1;
 template <typename REAL, bool INC>
 tridStatus_t ragged(const TridParams *ctx, const REAL *a, const REAL *b, const REAL *c, REAL *d,
                     REAL *u, const int64_t *offsets, int64_t nsys, int64_t max_len) {
-  if (!a || !b || !c || !d || !offsets || (INC && !u) || nsys < 0 || max_len < 0)
-    return TRID_STATUS_INVALID_VALUE;
+  if (!a || !b || !c || !d || !offsets || (INC && !u) || nsys < 0 || nsys > INT32_MAX ||
+      max_len < 0)
+    return TRID_STATUS_INVALID_VALUE;  // the plan's system list holds 32-bit indices
   int dev = 0;
   tridStatus_t s = check_device(&dev);
   if (s != TRID_STATUS_SUCCESS) return s;
@@ -47,7 +49,8 @@ tridStatus_t ragged(const TridParams *ctx, const REAL *a, const REAL *b, const R
 tridStatus_t ragged_mixed(const TridParams *ctx, const double *a, const double *b,
                           const double *c, double *d, const int64_t *offsets, int64_t nsys,
                           int64_t max_len, int refine) {
-  if (!a || !b || !c || !d || !offsets || nsys < 0 || max_len < 0 || refine < 0 || refine > 8)
+  if (!a || !b || !c || !d || !offsets || nsys < 0 || nsys > INT32_MAX || max_len < 0 ||
+      refine < 0 || refine > 8)
     return TRID_STATUS_INVALID_VALUE;
   int dev = 0;
   tridStatus_t s = check_device(&dev);
