@@ -100,7 +100,8 @@ EXTERN_C tridStatus_t tridHipAutotuneClear(void);
  * systems). The matrix is factored and the sweeps run in FP32; the residual
  * d - A x and the accumulation x += dx are FP64. One pass over HBM: a, b, c,
  * d are read once and d is overwritten with the FP64 solution. Systems up to
- * 4096 rows (8 rows per lane, 512 lanes). */
+ * 8192 rows along a contiguous dimension (16 rows per lane, 512 lanes) and
+ * 4096 rows along a strided one (8 rows per lane). */
 EXTERN_C tridStatus_t tridDmtsvStridedBatchMixed(const TridParams *ctx, const double *a,
                                                  const double *b, const double *c, double *d,
                                                  int ndim, int solvedim, const int *dims,
@@ -112,8 +113,10 @@ EXTERN_C tridStatus_t tridDmtsvStridedBatchMixed(const TridParams *ctx, const do
  * A^T lam = w and writes lam (= dL/dd) and, when the pointers are not NULL,
  * dL/da = -lam * x[n-1], dL/db = -lam * x[n], dL/dc = -lam * x[n+1]; the
  * entries a[0] and c[N-1] of every system get 0 (they are unused by the
- * solve). x must be the forward solution. Systems up to one workgroup long
- * (16384 rows FP32, 8192 FP64 on contiguous systems). */
+ * solve). x must be the forward solution; it is read only for those three
+ * gradients and may be NULL when all three pointers are NULL (the solve for
+ * lam alone then moves the forward solve's bytes). Systems up to one
+ * workgroup long (16384 rows FP32, 8192 FP64 on contiguous systems). */
 EXTERN_C tridStatus_t tridDmtsvStridedBatchAdjoint(const TridParams *ctx, const double *a,
                                                    const double *b, const double *c,
                                                    const double *x, const double *w,

@@ -86,6 +86,20 @@ def test_inc_requires_u():
     assert st == 3
 
 
+def test_adjoint_needs_x_only_for_coefficient_gradients():
+    lib = _lib.lib()
+    p = C.c_void_p(16)
+    dims = _lib.int3([4, 4, 4])
+    # a coefficient gradient without the forward solution x
+    st = lib.tridDmtsvStridedBatchAdjoint(None, p, p, p, None, p, p, None, p, None, 3, 0,
+                                          dims, dims)
+    assert st == 3
+    # lam without w
+    st = lib.tridDmtsvStridedBatchAdjoint(None, p, p, p, None, None, p, None, None, None, 3,
+                                          0, dims, dims)
+    assert st == 3
+
+
 def test_workspace_bytes():
     lib = _lib.lib()
     assert lib.tridHipWorkspaceBytes(1, 0, 3, 1, _lib.int3([8, 4, 2])) == 8 * 8 * 4 * 2

@@ -37,7 +37,7 @@ def dev(x):
 # FP32 sweeps + FP64 refinement
 # --------------------------------------------------------------------------
 @pytest.mark.parametrize("dims", [[64, 48, 40], [37, 21, 19], [512, 3, 2], [9, 700, 2],
-                                  [1000]])
+                                  [1000], [8192, 3], [3, 4096]])
 def test_mixed_refinement_accuracy(dims):
     rng = np.random.default_rng(sum(dims))
     n = int(np.prod(dims))
@@ -196,6 +196,13 @@ def test_adjoint_vs_oracle(dims, dtype):
         lam_only = api.adjoint_strided_batch(*t, solvedim=sd, dims=dims, sync=True)
         assert lam_only[1] is None and lam_only[2] is None and lam_only[3] is None
         H.assert_norm_close(lam_only[0].cpu().numpy(), ref[0], tol=tol)
+        # lam alone does not read x: NULL x gives the same bits
+        no_x = api.adjoint_strided_batch(t[0], t[1], t[2], None, t[4], solvedim=sd, dims=dims,
+                                         sync=True)
+        assert np.array_equal(no_x[0].cpu().numpy(), lam_only[0].cpu().numpy())
+        with pytest.raises(ValueError):
+            api.adjoint_strided_batch(t[0], t[1], t[2], None, t[4], solvedim=sd, dims=dims,
+                                      grad_b=True)
         # duality <w, A^-1 d> = <A^-T w, d>
         lam = out[0].cpu().numpy().astype(np.float64)
         assert abs(np.dot(w, x) - np.dot(lam, d)) <= 50 * tol * np.abs(w).sum()

@@ -193,18 +193,22 @@ def adjoint_strided_batch(a, b, c, x, w, *, solvedim, dims=None, pads=None, offs
 
     Returns (lam, grad_a, grad_b, grad_c); lam = dL/dd is always computed,
     each gradient only if its output tensor is given (or True to allocate).
+    x (the forward solution) is read only for those gradients and may be None
+    when none is requested.
     """
-    dims = list(x.shape[::-1]) if dims is None else dims
-    pads = (list(x.shape[::-1]) if x.dim() == len(dims) else list(dims)) if pads is None else pads
-    lam = torch.empty_like(x) if lam is None else lam
+    dims = list(w.shape[::-1]) if dims is None else dims
+    pads = (list(w.shape[::-1]) if w.dim() == len(dims) else list(dims)) if pads is None else pads
+    lam = torch.empty_like(w) if lam is None else lam
     outs = []
     for g in (grad_a, grad_b, grad_c):
-        outs.append(torch.empty_like(x) if g is True else g)
-    arrs = [a, b, c, x, w, lam] + [g for g in outs if g is not None]
+        outs.append(torch.empty_like(w) if g is True else g)
+    if x is None and any(g is not None for g in outs):
+        raise ValueError("the coefficient gradients need the forward solution x")
+    arrs = [t for t in (a, b, c, x, w, lam) if t is not None] + [g for g in outs if g is not None]
     _validate(arrs, dims, pads, offset)
     use_torch_stream(stream)
     params = make_params((0, 0, 0), 1 if sync else 0)
-    name = "tridDmtsvStridedBatchAdjoint" if x.dtype == torch.float64 else \
+    name = "tridDmtsvStridedBatchAdjoint" if w.dtype == torch.float64 else \
         "tridSmtsvStridedBatchAdjoint"
     ptr = lambda t: None if t is None else _ptr(t, offset)
     st = getattr(_lib.lib(), name)(C.byref(params), ptr(a), ptr(b), ptr(c), ptr(x), ptr(w),

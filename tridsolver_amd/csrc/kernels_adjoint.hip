@@ -13,6 +13,8 @@
 // every element is still read once), and the neighbour chunks' boundary x
 // values come through LDS. HBM traffic: read a, b, c, w, x; write lam and
 // each requested gradient (72 B per point at FP64 with all three gradients).
+// When no coefficient gradient is requested (GRAD false) x is not read: the
+// solve for lam alone moves the forward solve's 40 B per point.
 
 #include <algorithm>
 #include <atomic>
@@ -29,7 +31,7 @@ struct AdjBounds {
   static constexpr int threads = sizeof(REAL) * M >= 64 ? 512 : 1024;
 };
 
-template <typename REAL, int M, bool CONTIG>
+template <typename REAL, int M, bool CONTIG, bool GRAD>
 __global__ __launch_bounds__((AdjBounds<REAL, M>::threads)) void adjoint_kernel(
     Geom g, int P, int W, int G, const REAL *__restrict__ a, const REAL *__restrict__ b,
     const REAL *__restrict__ c, const REAL *__restrict__ x, const REAL *__restrict__ w,
@@ -46,14 +48,19 @@ __global__ __launch_bounds__((AdjBounds<REAL, M>::threads)) void adjoint_kernel(
   hyb::fetch<M, CONTIG>(L, b, rb, REAL(1), 0, 0, L.N);
   hyb::fetch<M, CONTIG>(L, a, rc, REAL(0), +1, 1, L.N);  // sup of A^T: a[n+1]
   hyb::fetch<M, CONTIG>(L, w, rd, REAL(0), 0, 0, L.N);
-  hyb::fetch<M, CONTIG>(L, x, rx, REAL(0), 0, 0, L.N);
+  if (GRAD) hyb::fetch<M, CONTIG>(L, x, rx, REAL(0), 0, 0, L.N);
   hyb::to_chunk<M, CONTIG>(L, ra, tile);
   hyb::to_chunk<M, CONTIG>(L, rb, tile);
   hyb::to_chunk<M, CONTIG>(L, rc, tile);
   hyb::to_chunk<M, CONTIG>(L, rd, tile);
-  hyb::to_chunk<M, CONTIG>(L, rx, tile);
+  if (GRAD) hyb::to_chunk<M, CONTIG>(L, rx, tile);
   REAL l[M];
   hyb::solve_chunk<REAL, M>(L, ring, ra, rb, rc, rd, l);
+  if (!GRAD) {
+    hyb::to_fetch<M, CONTIG>(L, l, tile);
+    hyb::store<M, CONTIG>(L, lam, l);
+    return;
+  }
 
   // x of the neighbouring chunks' boundary rows
   REAL xprev = 0, xnext = 0;
@@ -98,14 +105,14 @@ __global__ __launch_bounds__((AdjBounds<REAL, M>::threads)) void adjoint_kernel(
   hyb::store<M, CONTIG>(L, lam, l);
 }
 
-template <typename REAL, int M, bool CONTIG>
+template <typename REAL, int M, bool CONTIG, bool GRAD>
 hipError_t launch_adjoint_m(const Geom &g, const HybridCfg &cfg, const REAL *a,
                             const REAL *b, const REAL *c, const REAL *x, const REAL *w,
                             REAL *lam, REAL *ga, REAL *gb, REAL *gc, hipStream_t st) {
   const int T = cfg.W * cfg.P;
   size_t lds = hyb::ring_bytes<REAL>(T, 3);
   if (CONTIG) lds += sizeof(REAL) * (size_t)T * (M + 1);
-  auto kern = adjoint_kernel<REAL, M, CONTIG>;
+  auto kern = adjoint_kernel<REAL, M, CONTIG, GRAD>;
   static LdsGrant lds_granted;
   {
     hipError_t e = ensure_lds(reinterpret_cast<const void *>(kern), lds, lds_granted);
@@ -173,13 +180,18 @@ template <typename REAL>
 hipError_t launch_adjoint(const Geom &g, const HybridCfg &cfg, const REAL *a, const REAL *b,
                           const REAL *c, const REAL *x, const REAL *w, REAL *lam, REAL *ga,
                           REAL *gb, REAL *gc, hipStream_t st) {
-  const bool contig = g.stride == 1;
-#define TRID_ADJ_CASE(MM)                                                                   \
-  case MM:                                                                                  \
-    return contig ? launch_adjoint_m<REAL, MM, true>(g, cfg, a, b, c, x, w, lam, ga, gb, gc, \
-                                                     st)                                    \
-                  : launch_adjoint_m<REAL, MM, false>(g, cfg, a, b, c, x, w, lam, ga, gb,   \
-                                                      gc, st);
+  const bool contig = g.stride == 1, grad = ga || gb || gc;
+#define TRID_ADJ_CASE(MM)                                                                  \
+  case MM:                                                                                 \
+    if (grad)                                                                              \
+      return contig ? launch_adjoint_m<REAL, MM, true, true>(g, cfg, a, b, c, x, w, lam, ga, \
+                                                             gb, gc, st)                   \
+                    : launch_adjoint_m<REAL, MM, false, true>(g, cfg, a, b, c, x, w, lam, \
+                                                              ga, gb, gc, st);             \
+    return contig ? launch_adjoint_m<REAL, MM, true, false>(g, cfg, a, b, c, x, w, lam, ga, \
+                                                            gb, gc, st)                    \
+                  : launch_adjoint_m<REAL, MM, false, false>(g, cfg, a, b, c, x, w, lam, ga, \
+                                                             gb, gc, st);
   switch (cfg.M) {
     TRID_ADJ_CASE(1)
     TRID_ADJ_CASE(2)

@@ -86,7 +86,8 @@ tridStatus_t adjoint(const TridParams *ctx, const REAL *a, const REAL *b, const 
   bool empty = false;
   tridStatus_t s = make_geom(ndim, solvedim, dims, pads, &g, &empty);
   if (s != TRID_STATUS_SUCCESS) return s;
-  if (!a || !b || !c || !x || !w || !lam) return TRID_STATUS_INVALID_VALUE;
+  // x is read only for the coefficient gradients
+  if (!a || !b || !c || !w || !lam || (!x && (ga || gb || gc))) return TRID_STATUS_INVALID_VALUE;
   int dev = 0;
   if ((s = check_device(&dev)) != TRID_STATUS_SUCCESS) return s;
   hipStream_t st = current_stream();
