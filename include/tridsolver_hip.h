@@ -48,7 +48,8 @@ EXTERN_C size_t tridHipWorkspaceBytes(int fp64, int inc, int ndim, int solvedim,
  * hybrid, 7 streamed strided hybrid with LDS storage, 8 wide strided hybrid,
  * 9 ragged mixed precision), and for the hybrid its
  * rows per lane M, lanes per system P, systems per workgroup W and XCD
- * grouping G (consecutive system groups dealt to one XCD; 1 = plain order). */
+ * grouping G (consecutive system groups dealt to one XCD; 1 = plain order).
+ * For the ragged solves: M, W = solve launches, G = workgroups per CU. */
 EXTERN_C tridStatus_t tridHipLastConfig(int *algorithm, int *M, int *P, int *W, int *G);
 
 /* Forces the hybrid shape for this thread's subsequent solves (M = 0 clears).
@@ -135,13 +136,15 @@ EXTERN_C tridStatus_t tridSmtsvStridedBatchAdjoint(const TridParams *ctx, const 
  * No counterpart in the reference (its API needs one length for all systems).
  * nsys independent systems stored back to back: system s occupies elements
  * offsets[s] .. offsets[s+1]-1 of a, b, c, d (offsets: nsys+1 int64 values in
- * device memory, non-decreasing; nsys <= 2^31 - 1). Within a system the data contract of
- * tridsolver.h holds (a[first] and c[last] unused). max_len is an upper bound
- * of every length, known to the caller (at most 8192); a
- * longer system found on the device is skipped and, for a synchronous call,
- * reported as TRID_STATUS_INVALID_VALUE. One persistent launch after four
- * small planning kernels; capturable. Needs tridHipRaggedWorkspaceBytes(nsys)
- * of the stream's workspace (reserve it before a hipGraph capture). */
+ * device memory, non-decreasing; nsys <= 2^31 - 1). Within a system the data
+ * contract of tridsolver.h holds (a[first] and c[last] unused). max_len is an
+ * upper bound of every length, known to the caller (at most 8192); a longer
+ * system found on the device is skipped and, for a synchronous call, reported
+ * as TRID_STATUS_INVALID_VALUE (an asynchronous call cannot report it). Four
+ * small planning kernels, then one persistent launch that takes every system
+ * from a work queue (tridHipForceRaggedConfig can split it in two); capturable.
+ * Needs tridHipRaggedWorkspaceBytes(nsys) of the stream's workspace (reserve
+ * it before a hipGraph capture). */
 EXTERN_C tridStatus_t tridDmtsvRaggedBatch(const TridParams *ctx, const double *a,
                                            const double *b, const double *c, double *d,
                                            const int64_t *offsets, int64_t nsys,
@@ -170,9 +173,12 @@ EXTERN_C tridStatus_t tridDmtsvRaggedBatchMixed(const TridParams *ctx, const dou
                                                 const int64_t *offsets, int64_t nsys,
                                                 int64_t max_len, int refinements);
 EXTERN_C size_t tridHipRaggedWorkspaceBytes(int64_t nsys);
-/* Forces the ragged kernel's rows per lane (4, 8 or 16) and
- * workgroups per CU for this thread's calls; M = 0 restores the tuned choice. */
-EXTERN_C tridStatus_t tridHipForceRaggedConfig(int M, int per_cu);
+/* Forces the ragged kernel's rows per lane (4, 8 or 16), workgroups per CU
+ * and solve launches (1; or 2: systems needing more than half a workgroup's
+ * lanes, then the rest in half-size workgroups, two per CU) for this thread's
+ * calls; M = 0 restores the tuned choice. The mixed ragged solve takes the
+ * same settings (its M = 4 and 8 always run in one launch). */
+EXTERN_C tridStatus_t tridHipForceRaggedConfig(int M, int per_cu, int launches);
 
 /* ---- measurement ---------------------------------------------------------
  * No counterpart in the reference. Streams `bytes` (a multiple of 16, both

@@ -338,14 +338,17 @@ def check_ragged(off, a, b, c, d, got, dtype, u0=None):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-@pytest.mark.parametrize("M", [0, 4, 8, 16])
-def test_ragged_vs_oracle(dtype, M):
+@pytest.mark.parametrize("M,launches", [(0, 1), (4, 1), (8, 1), (16, 1), (4, 2), (8, 2),
+                                        (16, 2)])
+def test_ragged_vs_oracle(dtype, M, launches):
+    """Default (tuned or untuned) and forced shapes, in one persistent solve
+    launch or split in two (long systems, then the rest)."""
     rng = np.random.default_rng(M + 1)
     max_len = 2048 if M == 4 and dtype == np.float64 else 4096
     lens = np.round(np.exp(rng.uniform(np.log(1), np.log(max_len), 3000))).astype(np.int64)
     lens[:5] = [0, 1, 2, 0, max_len]
     off, a, b, c, d = ragged_case(rng, lens, dtype)
-    ctx = api.forced_ragged(M) if M else None
+    ctx = api.forced_ragged(M, launches=launches) if M else None
     if ctx:
         ctx.__enter__()
     try:
@@ -355,6 +358,8 @@ def test_ragged_vs_oracle(dtype, M):
             u0 = None if tu is None else tu.cpu().numpy()
             api.ragged_batch(dev(a), dev(b), dev(c), td, dev(off), max_len=max_len, u=tu,
                              sync=True)
+            if M:
+                assert api.last_config()[:4:3] == (5, launches)  # (algorithm, W = launches)
             got = (tu if inc else td).cpu().numpy()
             if inc:
                 assert np.array_equal(td.cpu().numpy(), d)
@@ -440,25 +445,25 @@ def test_config_ragged_40k_mixed(refine):
           % (refine, err))
 
 
-@pytest.mark.parametrize("M", [0, 4, 8, 16])
-def test_ragged_mixed_vs_oracle(M):
+@pytest.mark.parametrize("M,launches", [(0, 1), (4, 1), (8, 1), (16, 1), (16, 2)])
+def test_ragged_mixed_vs_oracle(M, launches):
     """FP64 ragged batch by FP32 sweeps + FP64 refinement: within 1e-12 of the
     FP64 oracle after 2 refinements; 0 refinements is an FP32-accurate solve.
     M = 4 and 8 keep every operand in registers, M = 16 (the default) keeps
-    a and c in LDS."""
+    a and c in LDS, in one solve launch or split in two."""
     rng = np.random.default_rng(40 + M)
     max_len = {4: 2048, 8: 4096}.get(M, 8192)
     lens = np.round(np.exp(rng.uniform(np.log(1), np.log(max_len), 2000))).astype(np.int64)
     lens[:5] = [0, 1, 2, 0, max_len]
     off, a, b, c, d = ragged_case(rng, lens, np.float64)
-    ctx = api.forced_ragged(M) if M else None
+    ctx = api.forced_ragged(M, launches=launches) if M else None
     if ctx:
         ctx.__enter__()
     try:
         td = dev(d)
         api.ragged_batch_mixed(dev(a), dev(b), dev(c), td, dev(off), max_len=max_len,
                                refinements=2, sync=True)
-        assert api.last_config()[0] == 9
+        assert api.last_config()[:4:3] == (9, launches)
         check_ragged(off, a, b, c, d, td.cpu().numpy(), np.float64)
         t0 = dev(d)
         api.ragged_batch_mixed(dev(a), dev(b), dev(c), t0, dev(off), max_len=max_len,

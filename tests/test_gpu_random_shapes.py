@@ -111,8 +111,8 @@ def test_random_shape_mixed(case):
 @pytest.mark.parametrize("case", range(30))
 def test_random_ragged(case):
     """Random ragged batches: 1..300 systems of 0..8192 rows (log-uniform, with
-    empty ones), FP32/FP64, plain/Inc, default or forced rows per lane, and
-    the mixed-precision ragged solve for FP64."""
+    empty ones), FP32/FP64, plain/Inc, default or forced rows per lane in one
+    or two solve launches, and the mixed-precision ragged solve for FP64."""
     rng = np.random.default_rng(3000 + case)
     S = int(rng.integers(1, 301))
     lens = np.round(np.exp(rng.uniform(0, np.log(8192), S))).astype(np.int64)
@@ -120,6 +120,7 @@ def test_random_ragged(case):
     dtype = np.float64 if rng.random() < 0.6 else np.float32
     inc = bool(rng.random() < 0.3)
     M = [0, 4, 8, 16][int(rng.integers(0, 4))]
+    launches = int(rng.integers(1, 3))
     max_len = int(lens.max())
     limit = {4: 4096, 8: 8192, 16: 8192}.get(M, 8192)
     if M == 4:  # 4 rows per lane hold up to 4096 rows (1024 lanes)
@@ -131,7 +132,7 @@ def test_random_ragged(case):
     u = rng.uniform(-1, 1, n).astype(dtype) if inc else None
     dev = lambda x: torch.from_numpy(np.ascontiguousarray(x)).cuda()
     td, tu = dev(d), (dev(u) if inc else None)
-    ctx = api.forced_ragged(M) if M else None
+    ctx = api.forced_ragged(M, launches=launches) if M else None
     if ctx:
         ctx.__enter__()
     try:

@@ -86,7 +86,7 @@ SIGNATURES = {
     "tridHipAutotuneSave": ([C.c_char_p], I),
     "tridHipAutotuneLoad": ([C.c_char_p], I),
     "tridHipAutotuneClear": ([], I),
-    "tridHipForceRaggedConfig": ([I, I], I),
+    "tridHipForceRaggedConfig": ([I, I, I], I),
     # include/tridsolver_adi.h
     "tridAdiPreprocD": ([D] + [P] * 11 + [I, I, I], I),
     "tridAdiPreprocS": ([F] + [P] * 11 + [I, I, I], I),

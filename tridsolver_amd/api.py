@@ -255,18 +255,19 @@ def ragged_batch_mixed(a, b, c, d, offsets, *, max_len, refinements=2, sync=Fals
 
 
 class forced_ragged:
-    """Context manager forcing the ragged kernel's rows per lane and
-    workgroups per CU for this thread's calls."""
+    """Context manager forcing the ragged kernel's rows per lane, workgroups
+    per CU and solve launches (1, or 2 with the short systems apart) for this
+    thread's calls."""
 
-    def __init__(self, M, per_cu=1):
-        self.cfg = (M, per_cu)
+    def __init__(self, M, per_cu=1, launches=1):
+        self.cfg = (M, per_cu, launches)
 
     def __enter__(self):
         check(_lib.lib().tridHipForceRaggedConfig(*self.cfg), "tridHipForceRaggedConfig")
         return self
 
     def __exit__(self, *exc):
-        _lib.lib().tridHipForceRaggedConfig(0, 1)
+        _lib.lib().tridHipForceRaggedConfig(0, 1, 1)
         return False
 
 

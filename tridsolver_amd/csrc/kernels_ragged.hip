@@ -1,7 +1,8 @@
 // kernels_ragged.hip - ragged batch: S independent systems of uneven lengths
 // stored back to back (system s = rows offsets[s] .. offsets[s+1]-1 of a, b,
-// c, d), solved in one persistent launch. No counterpart in the reference,
-// whose API needs equal lengths; per system the result is the plain solve's.
+// c, d), solved by persistent workgroups that take work items from a queue.
+// No counterpart in the reference, whose API needs equal lengths; per system
+// the result is the plain solve's.
 //
 // Plan (four tiny kernels, all stream-ordered and capturable):
 //   reset     zero the class counters, the work counter and the error flag
@@ -10,13 +11,15 @@
 //   prefix    per class: first list slot and first work item; items are
 //             ordered longest class first (largest-first scheduling)
 //   scatter   list[slot] = s, slots reserved per workgroup and class
-// Solve: two persistent launches. Systems needing more than T/2 lanes are
-// solved by one T-thread workgroup per CU (times `per_cu`), the rest by two
-// T/2-thread workgroups per CU. Each workgroup repeatedly takes the next work
-// item of its range from an atomic counter -- W systems of one class, P
-// lanes each, the Thomas-PCR hybrid of hybrid_core.hpp -- and leaves when the
-// counter passes the range's last item, so every wave exits. Systems longer
-// than M*T set the error flag and are skipped.
+// Solve: one persistent launch of T-thread workgroups, one per CU (times
+// `per_cu`), over every work item; or, split in two, systems needing more
+// than T/2 lanes by that launch and the rest by a second launch of two
+// T/2-thread workgroups per CU (one loads while the other computes). Each
+// workgroup repeatedly takes the next work item of its range from an atomic
+// counter -- W systems of one class, P lanes each, the Thomas-PCR hybrid of
+// hybrid_core.hpp -- and leaves when the counter passes the range's last
+// item, so every wave exits. Systems longer than M*T set the error flag and
+// are skipped.
 
 #include <algorithm>
 #include <atomic>
@@ -196,7 +199,8 @@ struct RgBounds {
 // One persistent launch over a range of items: LONG = the items whose systems
 // need more than Tshort lanes (workgroups of T = Tmax threads, one per CU),
 // else the rest (T = Tshort, two workgroups per CU, so that one loads while
-// the other computes).
+// the other computes). A single launch plans with Tshort = Tmax: no item is
+// long, and the LONG = false launch at T = Tmax takes them all.
 template <typename REAL, int M, bool INC, int T, bool LONG>
 __global__ __launch_bounds__(T) void rg_solve(
     const REAL *__restrict__ a, const REAL *__restrict__ b, const REAL *__restrict__ c,
@@ -260,16 +264,27 @@ void launch_plan(const int64_t *off, int64_t S, int M, int Tmax, int Tshort, voi
 
 template <typename REAL, int M, bool INC>
 hipError_t launch_m(const REAL *a, const REAL *b, const REAL *c, REAL *d, REAL *u,
-                    const int64_t *off, int64_t S, int per_cu, void *ws, hipStream_t st,
-                    int dev) {
+                    const int64_t *off, int64_t S, int per_cu, int launches, void *ws,
+                    hipStream_t st, int dev) {
   constexpr int Tmax = RgBounds<REAL, M>::threads;
   constexpr int Tshort = Tmax / 2;
   RaggedPlan *plan = static_cast<RaggedPlan *>(ws);
   int *list = reinterpret_cast<int *>(static_cast<unsigned char *>(ws) + 512);
-  launch_plan(off, S, M, Tmax, Tshort, ws, st);
   int cus = 256;
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   const int per = std::max(1, per_cu);
+  if (launches == 1) {  // every item in one launch of Tmax-thread workgroups
+    launch_plan(off, S, M, Tmax, Tmax, ws, st);
+    const size_t lds = sizeof(REAL) * (size_t)Tmax * (M + 1) + hyb::ring_bytes<REAL>(Tmax, 3);
+    auto kern = rg_solve<REAL, M, INC, Tmax, false>;
+    static LdsGrant lds_granted;
+    hipError_t e = ensure_lds(reinterpret_cast<const void *>(kern), lds, lds_granted);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kern, dim3((unsigned)(cus * per)), dim3(Tmax), lds, st, a, b, c, d, u,
+                       off, list, plan);
+    return hipGetLastError();
+  }
+  launch_plan(off, S, M, Tmax, Tshort, ws, st);
   {  // the long systems first (an empty range exits at once)
     const size_t lds = sizeof(REAL) * (size_t)Tmax * (M + 1) + hyb::ring_bytes<REAL>(Tmax, 3);
     auto kern = rg_solve<REAL, M, INC, Tmax, true>;
@@ -357,10 +372,11 @@ hipError_t launch_mixed_m(const double *a, const double *b, const double *c, dou
 }
 
 // M = 16 with the FP64 a and c operands in LDS (mixed_contig_lds): 8192 rows
-// in 512 threads. Same split as the FP64 solve: systems needing more than
-// 256 lanes in one 512-thread workgroup per CU, the rest in 256-thread
-// workgroups, two per CU (LDS 2 tiles x T*(M+1) doubles + 2 ring slots of
-// 2 doubles per thread: 152 KiB at 512 threads, 76 KiB at 256).
+// in 512 threads. One launch of 512-thread workgroups, one per CU, or the
+// FP64 solve's split: systems needing more than 256 lanes by that launch, the
+// rest by 256-thread workgroups, two per CU (LDS 2 tiles x T*(M+1) doubles +
+// 2 ring slots of 2 doubles per thread: 152 KiB at 512 threads, 76 KiB at
+// 256).
 constexpr int RG_MIXED_LDS_M = 16;
 
 template <int T>
@@ -424,15 +440,20 @@ hipError_t launch_rg_mixed_lds(const double *a, const double *b, const double *c
 }
 
 hipError_t launch_mixed_lds(const double *a, const double *b, const double *c, double *d,
-                            const int64_t *off, int64_t S, int per_cu, int refine, void *ws,
-                            hipStream_t st, int dev) {
+                            const int64_t *off, int64_t S, int per_cu, int launches, int refine,
+                            void *ws, hipStream_t st, int dev) {
   constexpr int Tmax = 512, Tshort = 256;
   RaggedPlan *plan = static_cast<RaggedPlan *>(ws);
   int *list = reinterpret_cast<int *>(static_cast<unsigned char *>(ws) + 512);
-  launch_plan(off, S, RG_MIXED_LDS_M, Tmax, Tshort, ws, st);
   int cus = 256;
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   const int per = std::max(1, per_cu);
+  if (launches == 1) {  // no long range: the LONG = false launch takes every item
+    launch_plan(off, S, RG_MIXED_LDS_M, Tmax, Tmax, ws, st);
+    return launch_rg_mixed_lds<Tmax, false>(a, b, c, d, off, list, plan,
+                                            (unsigned)(cus * per), refine, st);
+  }
+  launch_plan(off, S, RG_MIXED_LDS_M, Tmax, Tshort, ws, st);
   hipError_t e = launch_rg_mixed_lds<Tmax, true>(a, b, c, d, off, list, plan,
                                                  (unsigned)(cus * per), refine, st);
   if (e != hipSuccess) return e;
@@ -448,12 +469,14 @@ int ragged_mixed_max_len(int M) {
 }
 
 hipError_t launch_ragged_mixed(int M, const double *a, const double *b, const double *c,
-                               double *d, const int64_t *off, int64_t S, int per_cu, int refine,
-                               void *ws, hipStream_t st, int dev) {
+                               double *d, const int64_t *off, int64_t S, int per_cu,
+                               int launches, int refine, void *ws, hipStream_t st, int dev) {
+  if (launches != 1 && launches != 2) return hipErrorInvalidValue;
   switch (M) {
     case 4: return launch_mixed_m<4>(a, b, c, d, off, S, per_cu, refine, ws, st, dev);
     case 8: return launch_mixed_m<8>(a, b, c, d, off, S, per_cu, refine, ws, st, dev);
-    case RG_MIXED_LDS_M: return launch_mixed_lds(a, b, c, d, off, S, per_cu, refine, ws, st, dev);
+    case RG_MIXED_LDS_M:
+      return launch_mixed_lds(a, b, c, d, off, S, per_cu, launches, refine, ws, st, dev);
     default: return hipErrorInvalidValue;
   }
 }
@@ -468,13 +491,15 @@ int ragged_max_len(int fp64, int M) {
 
 template <typename REAL, bool INC>
 hipError_t launch_ragged(int M, const REAL *a, const REAL *b, const REAL *c, REAL *d, REAL *u,
-                         const int64_t *off, int64_t S, int per_cu, void *ws, hipStream_t st,
-                         int dev) {
+                         const int64_t *off, int64_t S, int per_cu, int launches, void *ws,
+                         hipStream_t st, int dev) {
   static_assert(sizeof(RaggedPlan) <= 512, "plan header");
+  if (launches != 1 && launches != 2) return hipErrorInvalidValue;
   switch (M) {
-    case 4: return launch_m<REAL, 4, INC>(a, b, c, d, u, off, S, per_cu, ws, st, dev);
-    case 8: return launch_m<REAL, 8, INC>(a, b, c, d, u, off, S, per_cu, ws, st, dev);
-    case 16: return launch_m<REAL, 16, INC>(a, b, c, d, u, off, S, per_cu, ws, st, dev);
+    case 4: return launch_m<REAL, 4, INC>(a, b, c, d, u, off, S, per_cu, launches, ws, st, dev);
+    case 8: return launch_m<REAL, 8, INC>(a, b, c, d, u, off, S, per_cu, launches, ws, st, dev);
+    case 16:
+      return launch_m<REAL, 16, INC>(a, b, c, d, u, off, S, per_cu, launches, ws, st, dev);
     default: return hipErrorInvalidValue;
   }
 }
@@ -488,7 +513,7 @@ hipError_t ragged_error_flag(void *ws, int *flag) {
 #define TRID_RG_INST(REAL, INC)                                                              \
   template hipError_t launch_ragged<REAL, INC>(int, const REAL *, const REAL *, const REAL *, \
                                                REAL *, REAL *, const int64_t *, int64_t, int, \
-                                               void *, hipStream_t, int);
+                                               int, void *, hipStream_t, int);
 TRID_RG_INST(float, false)
 TRID_RG_INST(float, true)
 TRID_RG_INST(double, false)

@@ -13,7 +13,7 @@
 namespace trid {
 namespace {
 
-thread_local int tl_ragged_m = 0, tl_ragged_per_cu = 1;
+thread_local int tl_ragged_m = 0, tl_ragged_per_cu = 1, tl_ragged_launches = 1;
 
 template <typename REAL, bool INC>
 tridStatus_t ragged(const TridParams *ctx, const REAL *a, const REAL *b, const REAL *c, REAL *d,
@@ -26,18 +26,18 @@ tridStatus_t ragged(const TridParams *ctx, const REAL *a, const REAL *b, const R
   if (s != TRID_STATUS_SUCCESS) return s;
   hipStream_t st = current_stream();
   if (nsys == 0) return finish(ctx, st, hipSuccess);
-  int M = tl_ragged_m, per_cu = tl_ragged_per_cu;
-  if (M == 0) ragged_tuned_config<REAL>(dev, max_len, &M, &per_cu);
+  int M = tl_ragged_m, per_cu = tl_ragged_per_cu, launches = tl_ragged_launches;
+  if (M == 0) ragged_tuned_config<REAL>(dev, max_len, &M, &per_cu, &launches);
   const int supported = ragged_max_len(sizeof(REAL) == 8, M);
   if (supported == 0 || max_len > supported) return TRID_STATUS_INVALID_VALUE;
   void *ws = nullptr;
   if ((s = get_workspace(dev, st, ragged_workspace_bytes(nsys), &ws)) != TRID_STATUS_SUCCESS)
     return s;
   HybridCfg rec;
-  rec.M = M; rec.P = 0; rec.W = 0; rec.G = per_cu;
+  rec.M = M; rec.P = 0; rec.W = launches; rec.G = per_cu;
   record_config(5, &rec);
-  s = finish(ctx, st, launch_ragged<REAL, INC>(M, a, b, c, d, u, offsets, nsys, per_cu, ws, st,
-                                              dev));
+  s = finish(ctx, st, launch_ragged<REAL, INC>(M, a, b, c, d, u, offsets, nsys, per_cu, launches,
+                                              ws, st, dev));
   if (s == TRID_STATUS_SUCCESS && (ctx == nullptr || ctx->sync != 0)) {
     int flag = 0;
     if (ragged_error_flag(ws, &flag) != hipSuccess) return TRID_STATUS_EXECUTION_FAILED;
@@ -66,10 +66,12 @@ tridStatus_t ragged_mixed(const TridParams *ctx, const double *a, const double *
   if ((s = get_workspace(dev, st, ragged_workspace_bytes(nsys), &ws)) != TRID_STATUS_SUCCESS)
     return s;
   HybridCfg rec;
-  rec.M = M; rec.P = 0; rec.W = 0; rec.G = tl_ragged_per_cu;
+  // M = 4 and 8 always run in one launch
+  const int launches = M == 16 ? tl_ragged_launches : 1;
+  rec.M = M; rec.P = 0; rec.W = launches; rec.G = tl_ragged_per_cu;
   record_config(9, &rec);
   s = finish(ctx, st, launch_ragged_mixed(M, a, b, c, d, offsets, nsys, tl_ragged_per_cu,
-                                          refine, ws, st, dev));
+                                          launches, refine, ws, st, dev));
   if (s == TRID_STATUS_SUCCESS && (ctx == nullptr || ctx->sync != 0)) {
     int flag = 0;
     if (ragged_error_flag(ws, &flag) != hipSuccess) return TRID_STATUS_EXECUTION_FAILED;
@@ -183,15 +185,19 @@ EXTERN_C tridStatus_t tridSmtsvRaggedBatchInc(const TridParams *ctx, const float
   return ragged<float, true>(ctx, a, b, c, d, u, offsets, nsys, max_len);
 }
 
-EXTERN_C tridStatus_t tridHipForceRaggedConfig(int M, int per_cu) {
+EXTERN_C tridStatus_t tridHipForceRaggedConfig(int M, int per_cu, int launches) {
   if (M == 0) {
     tl_ragged_m = 0;
     tl_ragged_per_cu = 1;
+    tl_ragged_launches = 1;
     return TRID_STATUS_SUCCESS;
   }
-  if ((M != 4 && M != 8 && M != 16) || per_cu < 1 || per_cu > 8) return TRID_STATUS_INVALID_VALUE;
+  if ((M != 4 && M != 8 && M != 16) || per_cu < 1 || per_cu > 8 ||
+      (launches != 1 && launches != 2))
+    return TRID_STATUS_INVALID_VALUE;
   tl_ragged_m = M;
   tl_ragged_per_cu = per_cu;
+  tl_ragged_launches = launches;
   return TRID_STATUS_SUCCESS;
 }
 

@@ -107,14 +107,14 @@ hipError_t launch_adjoint(const Geom &g, const HybridCfg &cfg, const REAL *a, co
 // ---- ragged batch (kernels_ragged.hip) ------------------------------------
 int ragged_mixed_max_len(int M);
 hipError_t launch_ragged_mixed(int M, const double *a, const double *b, const double *c,
-                               double *d, const int64_t *off, int64_t S, int per_cu, int refine,
-                               void *ws, hipStream_t st, int dev);
+                               double *d, const int64_t *off, int64_t S, int per_cu,
+                               int launches, int refine, void *ws, hipStream_t st, int dev);
 size_t ragged_workspace_bytes(int64_t S);
 int ragged_max_len(int fp64, int M);
 template <typename REAL, bool INC>
 hipError_t launch_ragged(int M, const REAL *a, const REAL *b, const REAL *c, REAL *d, REAL *u,
-                         const int64_t *off, int64_t S, int per_cu, void *ws, hipStream_t st,
-                         int dev);
+                         const int64_t *off, int64_t S, int per_cu, int launches, void *ws,
+                         hipStream_t st, int dev);
 hipError_t ragged_error_flag(void *ws, int *flag);
 
 // The dynamic-LDS limit granted to one kernel, per device (the attribute is

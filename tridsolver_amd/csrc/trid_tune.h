@@ -13,9 +13,10 @@ bool tuned_config(int dev, const Geom &g, HybridCfg *cfg);
 // Shape of the fused FP32/FP64-refinement kernel for g.
 bool mixed_tuned_config(int dev, const Geom &g, HybridCfg *cfg);
 
-// Rows per lane M and workgroups per CU of the ragged persistent kernel for
-// systems up to max_len long.
+// Rows per lane M, workgroups per CU and solve launches (1, or 2 with the
+// short systems apart) of the ragged persistent kernel for systems up to
+// max_len long.
 template <typename REAL>
-void ragged_tuned_config(int dev, int64_t max_len, int *M, int *per_cu);
+void ragged_tuned_config(int dev, int64_t max_len, int *M, int *per_cu, int *launches);
 
 }  // namespace trid

@@ -47,7 +47,8 @@ struct Key {
 };
 
 struct Choice {
-  int algo;  // 1 lane, 3 hybrid (solve); 2 mixed; 5 ragged (cfg.M, cfg.G = per_cu)
+  int algo;  // 1 lane, 3 hybrid (solve); 2 mixed; 5 ragged (cfg.M, cfg.G = per_cu,
+             // cfg.V = solve launches)
   HybridCfg cfg;
   float ms;
 };
@@ -357,15 +358,17 @@ bool mixed_tuned_config(int dev, const Geom &g, HybridCfg *cfg) {
 }
 
 template <typename REAL>
-void ragged_tuned_config(int dev, int64_t max_len, int *M, int *per_cu) {
+void ragged_tuned_config(int dev, int64_t max_len, int *M, int *per_cu, int *launches) {
   Geom g{max_len, 1, 0, 0, 0, 0};
   const Key k = key_of(dev, KIND_RAGGED, sizeof(REAL) == 8, 0, g);
   Choice c;
   if (lookup(k, &c) && c.algo == 5 && ragged_max_len(sizeof(REAL) == 8, c.cfg.M) >= max_len) {
     *M = c.cfg.M;
     *per_cu = c.cfg.G;
+    *launches = c.cfg.V == 2 ? 2 : 1;
     return;
   }
+  *launches = 1;
   // untuned: 16 rows per lane where systems may exceed 4096 rows (the tuned
   // choice on the 40,000-system batch of lengths up to 8192, FP64 and FP32:
   // 0.51 against 0.54 ms and 0.333 against 0.348 ms at 8 rows), else 8
@@ -398,18 +401,23 @@ tridStatus_t tune_ragged(int dev, const int64_t *offsets, int64_t nsys, int64_t 
   Choice best;
   best.algo = 0;
   best.ms = INFINITY;
+  // One persistent solve launch. Splitting off the short systems into a
+  // second launch of half-size workgroups measured 2 % (FP64) and 3 % (FP32)
+  // faster on 40,000 systems of 8-8192 rows; it stays a forced option
+  // (tridHipForceRaggedConfig) so that the batch is one launch by default.
   for (int M : {4, 8, 16})
     for (int per_cu : {1, 2}) {
       if (ragged_max_len(sizeof(REAL) == 8, M) < max_len) continue;
       const float ms = time_on(st, [&] {
-        return launch_ragged<REAL, false>(M, a, b, c, d, nullptr, offsets, nsys, per_cu, ws, st,
-                                          dev);
+        return launch_ragged<REAL, false>(M, a, b, c, d, nullptr, offsets, nsys, per_cu, 1, ws,
+                                          st, dev);
       });
       if (ms < best.ms) {
         best.ms = ms;
         best.algo = 5;
         best.cfg.M = M;
         best.cfg.G = per_cu;
+        best.cfg.V = 1;
       }
     }
   (void)hipStreamSynchronize(st);
@@ -424,8 +432,8 @@ template bool tuned_config<float, false>(int, const Geom &, HybridCfg *);
 template bool tuned_config<float, true>(int, const Geom &, HybridCfg *);
 template bool tuned_config<double, false>(int, const Geom &, HybridCfg *);
 template bool tuned_config<double, true>(int, const Geom &, HybridCfg *);
-template void ragged_tuned_config<float>(int, int64_t, int *, int *);
-template void ragged_tuned_config<double>(int, int64_t, int *, int *);
+template void ragged_tuned_config<float>(int, int64_t, int *, int *, int *);
+template void ragged_tuned_config<double>(int, int64_t, int *, int *, int *);
 
 }  // namespace trid
 
