@@ -133,17 +133,19 @@ template <typename REAL, int M, bool CONTIG>
 __device__ __forceinline__ void adi_coefs(const hyb::Lane &L, const AdiCoef<REAL> &k,
                                           REAL (&ra)[M], REAL (&rb)[M], REAL (&rc)[M]) {
   const REAL ca = REAL(-0.5) * k.lambda, cb = REAL(1) + k.lambda;
+  // the contiguous sweep is the x sweep
+  const int sd = CONTIG ? 0 : k.sd;
   // coordinates of the system; the row index runs along direction sd
   int ci, cj, ck;
-  if (k.sd == 0) { cj = (int)L.i0; ck = (int)L.i1; ci = 0; }
-  else if (k.sd == 1) { ci = (int)L.i0; ck = (int)L.i1; cj = 0; }
+  if (sd == 0) { cj = (int)L.i0; ck = (int)L.i1; ci = 0; }
+  else if (sd == 1) { ci = (int)L.i0; ck = (int)L.i1; cj = 0; }
   else { ci = (int)L.i0; cj = (int)L.i1; ck = 0; }
 #pragma unroll
   for (int r = 0; r < M; ++r) {
     const int64_t row = (int64_t)L.p * M + r;
     int i = ci, j = cj, kk = ck;
-    if (k.sd == 0) i = (int)row;
-    else if (k.sd == 1) j = (int)row;
+    if (sd == 0) i = (int)row;
+    else if (sd == 1) j = (int)row;
     else kk = (int)row;
     const bool pad = !L.active || row >= L.N;
     const bool edge = i == 0 || i == k.nx - 1 || j == 0 || j == k.ny - 1 || kk == 0 ||
@@ -177,20 +179,31 @@ __global__ __launch_bounds__((AdiBounds<REAL, M>::threads)) void adi_solve_kerne
   hyb::Ring ring{smem_raw + tile_bytes, (int)(3 * sizeof(REAL) * L.T), 0};
   REAL ra[M], rb[M], rc[M], rd[M];
   if constexpr (RHS) {
+    // Every lane loads all of its rows unconditionally, from addresses kept
+    // inside the grid: rows past N read row N-1, lanes past the last line
+    // read line 0, and lines on the boundary read their own line as the
+    // "neighbours". Those values only feed rows whose right-hand side is 0
+    // below, so no load needs a branch around it.
+    // Offsets are 32-bit byte offsets from u (the launch checks that the
+    // grid is below 4 GiB).
     REAL uc[M], ym[M], yp[M], zm[M], zp[M];
-    hyb::fetch<M, true>(L, u, uc, REAL(0), 0, 0, L.N);
     const int j = (int)L.i0, k = (int)L.i1;  // the x line (j, k)
     const bool line_inner = L.active && j > 0 && j < kc.ny - 1 && k > 0 && k < kc.nz - 1;
-    const int64_t sy = kc.nx, sz = (int64_t)kc.nx * kc.ny;
+    const uint32_t sy = line_inner ? (uint32_t)(kc.nx * sizeof(REAL)) : 0u;
+    const uint32_t sz = line_inner ? (uint32_t)((uint32_t)kc.nx * kc.ny * sizeof(REAL)) : 0u;
+    const uint32_t lbase = L.active ? (uint32_t)L.base : 0u;
+    const int n = (int)L.N;
+    const char *ub = reinterpret_cast<const char *>(u);
+    auto ld = [ub](uint32_t off) { return *reinterpret_cast<const REAL *>(ub + off); };
 #pragma unroll
     for (int r = 0; r < M; ++r) {
-      const int64_t row = hyb::fetch_row<M, true>(L, r);
-      const bool ok = line_inner && row < L.N;
-      const int64_t o = L.base + row;
-      ym[r] = ok ? u[o - sy] : REAL(0);
-      yp[r] = ok ? u[o + sy] : REAL(0);
-      zm[r] = ok ? u[o - sz] : REAL(0);
-      zp[r] = ok ? u[o + sz] : REAL(0);
+      const int row = (int)hyb::fetch_row<M, true>(L, r);
+      const uint32_t o = (lbase + (uint32_t)(row < n ? row : n - 1)) * (uint32_t)sizeof(REAL);
+      uc[r] = ld(o);
+      ym[r] = ld(o - sy);
+      yp[r] = ld(o + sy);
+      zm[r] = ld(o - sz);
+      zp[r] = ld(o + sz);
     }
     // x neighbours: the line's u in the tile, row order
 #pragma unroll
@@ -214,8 +227,28 @@ __global__ __launch_bounds__((AdiBounds<REAL, M>::threads)) void adi_solve_kerne
   REAL x[M];
   hyb::solve_chunk<REAL, M>(L, ring, ra, rb, rc, rd, x);
   hyb::to_fetch<M, CONTIG>(L, x, tile);
-  if (INC) hyb::store_add<M, CONTIG>(L, u, x);
-  else hyb::store<M, CONTIG>(L, d, x);
+  if constexpr (RHS) {  // 32-bit byte offsets, as the loads
+    if (!L.active) return;
+    const int n = (int)L.N;
+    const uint32_t lbase = (uint32_t)L.base;
+    char *db = reinterpret_cast<char *>(d);
+    auto st = [&](int r) {
+      const uint32_t row = (uint32_t)hyb::fetch_row<M, true>(L, r);
+      *reinterpret_cast<REAL *>(db + (lbase + row) * (uint32_t)sizeof(REAL)) = x[r];
+    };
+    if ((int64_t)P * M == L.N) {  // every row of every lane lies inside the line
+#pragma unroll
+      for (int r = 0; r < M; ++r) st(r);
+    } else {
+#pragma unroll
+      for (int r = 0; r < M; ++r)
+        if ((int)hyb::fetch_row<M, true>(L, r) < n) st(r);
+    }
+  } else if (INC) {
+    hyb::store_add<M, CONTIG>(L, u, x);
+  } else {
+    hyb::store<M, CONTIG>(L, d, x);
+  }
 }
 
 template <typename REAL, int M, bool INC, bool CONTIG, bool RHS = false>
@@ -324,19 +357,28 @@ tridStatus_t adi_sweep(int dev, REAL lambda, REAL *d, REAL *u, int nx, int ny, i
   const bool contig = sd == 0;
   hipError_t e;
   if (RHS && !contig) return TRID_STATUS_INTERNAL_ERROR;
-  // FP32 x lines at 16 rows per lane would spill with the right-hand side
-  // in the sweep: there preproc writes du and the plain sweep reads it
+  // The right-hand side is computed in the sweep unless FP32 x lines are
+  // at 16 rows per lane (they would spill) or the grid is 4 GiB or more (the
+  // sweep's loads use 32-bit byte offsets): there preproc writes du and the
+  // plain sweep reads it.
   constexpr bool RHS16 = RHS && sizeof(REAL) == 8;
-  if (RHS && !RHS16 && cfg.M == 16) {
+  const bool grid32 =
+      (uint64_t)nx * (uint64_t)ny * (uint64_t)nz * sizeof(REAL) < (uint64_t(1) << 32);
+  const bool rhs_here = RHS && grid32 && (cfg.M != 16 || RHS16);
+  if (RHS && !rhs_here) {
     tridStatus_t sp = preproc<REAL>(lambda, u, d, nullptr, nullptr, nullptr, nullptr, nullptr,
                                     nullptr, nullptr, nullptr, nullptr, nx, ny, nz);
     if (sp != TRID_STATUS_SUCCESS) return sp;
   }
-#define TRID_ADI_CASE(MM)                                                              \
-  case MM:                                                                             \
-    e = contig ? launch_adi_solve_m<REAL, MM, INC, true, (MM == 16 ? RHS16 : RHS)>(    \
-                     g, cfg, kc, d, u, st)                                             \
-               : launch_adi_solve_m<REAL, MM, INC, false>(g, cfg, kc, d, u, st);        \
+#define TRID_ADI_CASE(MM)                                                                  \
+  case MM:                                                                                 \
+    if (contig && rhs_here)                                                                \
+      e = launch_adi_solve_m<REAL, MM, INC, true, (MM == 16 ? RHS16 : RHS)>(g, cfg, kc, d, \
+                                                                              u, st);      \
+    else if (contig)                                                                       \
+      e = launch_adi_solve_m<REAL, MM, INC, true, false>(g, cfg, kc, d, u, st);            \
+    else                                                                                   \
+      e = launch_adi_solve_m<REAL, MM, INC, false>(g, cfg, kc, d, u, st);                  \
     break;
   switch (cfg.M) {
     TRID_ADI_CASE(1)
