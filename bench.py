@@ -12,9 +12,11 @@ Algorithmic bytes per grid point (the roofline's `achieved`): read a, b, c, d
 and write d = 5 x 8 = 40 B (FP64).
 
 Extra measurements in the same JSON line ("extras"; skipped by --quick):
-FP32 x/y/z; FP32 sweeps + FP64 refinement vs FP64 (time ratio and error);
-adjoint / forward time on 256^3 FP64; the ragged 40,000-system batch; the
-hipGraph-replayed ADI step on 256^3.
+FP32 x/y/z; FP32 sweeps + FP64 refinement vs FP64 (time ratio and error),
+for FP64-stored operands (40 B per point) and for FP32-stored ones (24 B per
+point, FP64 solution); adjoint / forward time on 256^3 FP64; the ragged
+40,000-system batch; the hipGraph-replayed ADI step on 256^3; one 8-way
+z-slab shard of the partitioned solve.
 
 Prints ONE JSON line (the driver's contract).
 """
@@ -50,8 +52,8 @@ def parse():
     ap.add_argument("--cpu-slab", type=int, default=64,
                     help="z-extent of the CPU baseline sample (n x n x slab)")
     ap.add_argument("--verbose", action="store_true")
-    ap.add_argument("--only", choices=["fp32", "mixed", "adjoint", "ragged", "adi", "partitioned",
-                                       "cpu-extras"],
+    ap.add_argument("--only", choices=["fp32", "mixed", "mixed-f32", "adjoint", "ragged", "adi",
+                                       "partitioned", "cpu-extras"],
                     help="run one secondary workload and print its JSON instead of the bench "
                          "line (for profiling)")
     return ap.parse_args()
@@ -297,6 +299,42 @@ def bench_mixed(torch, api, n, fp64_ms, tune):
     return out
 
 
+def bench_mixed_f32(torch, api, n, fp64_ms):
+    """FP64 solution of the n^3 system stored in FP32 (tridDmtsvStridedBatchMixedF32):
+    16 B read + 8 B written per point. Speed is compared with the FP64 solve
+    of the same size (fp64_ms, 40 B per point); accuracy with the FP64 solve
+    of the FP32 values promoted to FP64 (the same system)."""
+    n3 = n ** 3
+    dims = [n, n, n]
+    a, b, c, d = gen_gpu(torch, n3, torch.float32, seed=32)
+    a64, b64, c64, d64 = (t.double() for t in (a, b, c, d))
+    x = torch.empty(n3, dtype=torch.float64, device="cuda")
+    out = {}
+    for sd, name in enumerate("xyz"):
+        ref = d64.clone()
+        api.trid_strided_batch(a64, b64, c64, ref, solvedim=sd, dims=dims)
+        runs = {}
+        for nref in (1, 2):
+            api.mixed_f32_strided_batch(a, b, c, d, x, solvedim=sd, dims=dims, refinements=nref)
+            err = float((x - ref).abs().max() / ref.abs().max())
+            ms = event_ms(torch, lambda: api.mixed_f32_strided_batch(
+                a, b, c, d, x, solvedim=sd, dims=dims, refinements=nref), reps=10, warm=3)
+            runs[nref] = {"ms": round(ms, 4), "speedup_vs_fp64": round(fp64_ms[name] / ms, 3),
+                          "max_rel_err_vs_fp64": err, "config": api.last_config()}
+        del ref
+        nref = 1 if runs[1]["max_rel_err_vs_fp64"] <= 1e-12 else 2
+        best = runs[nref]
+        out[name] = {"ms": best["ms"], "gpts_s": round(n3 / best["ms"] / 1e6, 2),
+                     "alg_GBs": round(24.0 * n3 / best["ms"] / 1e6, 1), "refinements": nref,
+                     "speedup_vs_fp64": best["speedup_vs_fp64"],
+                     "max_rel_err_vs_fp64": best["max_rel_err_vs_fp64"],
+                     "config": best["config"],
+                     "by_refinements": {str(k): v for k, v in runs.items()}}
+    del a, b, c, d, a64, b64, c64, d64, x
+    torch.cuda.empty_cache()
+    return out
+
+
 def bench_adjoint(torch, api, n=256):
     n3 = n ** 3
     dims = [n, n, n]
@@ -447,7 +485,7 @@ def main():
     if args.only:
         api.use_torch_stream()
         tune = not args.no_tune
-        if args.only == "mixed":
+        if args.only in ("mixed", "mixed-f32"):
             a, b, c, d0 = gen_gpu(torch, n ** 3, torch.float64, seed=64)
             ref = {}
             for sd, name in enumerate("xyz"):
@@ -458,7 +496,8 @@ def main():
                     a, b, c, d, solvedim=sd, dims=[n] * 3))
             del a, b, c, d0, d
             torch.cuda.empty_cache()
-            res = bench_mixed(torch, api, n, ref, tune)
+            res = bench_mixed(torch, api, n, ref, tune) if args.only == "mixed" else \
+                bench_mixed_f32(torch, api, n, ref)
         else:
             run = {"fp32": lambda: bench_fp32_dirs(torch, api, n, tune),
                    "adjoint": lambda: bench_adjoint(torch, api),
@@ -536,6 +575,8 @@ def main():
         log("fp32: %s" % extras["fp32_per_direction"])
         extras["fp32_refined_vs_fp64"] = bench_mixed(torch, api, n, per_dir_ms, tune)
         log("mixed: %s" % extras["fp32_refined_vs_fp64"])
+        extras["fp32_stored_refined_vs_fp64"] = bench_mixed_f32(torch, api, n, per_dir_ms)
+        log("mixed, FP32-stored: %s" % extras["fp32_stored_refined_vs_fp64"])
         extras["adjoint_256"] = bench_adjoint(torch, api)
         log("adjoint: %s" % extras["adjoint_256"])
         extras["ragged_40k"] = bench_ragged(torch, api, tune)

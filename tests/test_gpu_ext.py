@@ -158,6 +158,99 @@ def test_mixed_lds_variant_padded():
         assert np.array_equal(got[~mask], arrs[3][~mask])
 
 
+# --------------------------------------------------------------------------
+# FP64 solution of systems stored in FP32 (tridDmtsvStridedBatchMixedF32)
+# --------------------------------------------------------------------------
+def _f32_case(rng, n):
+    return H.diag_dominant(rng, n, np.float32)
+
+
+@pytest.mark.parametrize("dims", [[64, 48, 40], [37, 21, 19], [512, 3, 2], [9, 700, 2],
+                                  [1000], [8192, 3], [3, 4096], [96, 80, 512]])
+def test_mixed_f32_vs_long_double_oracle(dims):
+    """The system is the FP32 values promoted to FP64: 0 refinements is
+    FP32-accurate, 1 and 2 are within 1e-12 of the long-double oracle's
+    solution of that system; d is left as it was."""
+    rng = np.random.default_rng(sum(dims) + 11)
+    n = int(np.prod(dims))
+    a, b, c, d = _f32_case(rng, n)
+    ta, tb, tc, td = (dev(t) for t in (a, b, c, d))
+    for sd in range(len(dims)):
+        xl = oracle.batch_ld(a, b, c, d, ndim=len(dims), solvedim=sd, dims=dims)
+        errs = []
+        for refine in (0, 1, 2):
+            x = api.mixed_f32_strided_batch(ta, tb, tc, td, solvedim=sd, dims=dims,
+                                            refinements=refine, sync=True)
+            assert api.last_config()[0] == 10
+            errs.append(H.norm_rel_err(x.cpu().numpy(), xl))
+        assert errs[0] <= 1e-5 and errs[1] <= 1e-12 and errs[2] <= 1e-12, (sd, errs)
+        assert errs[1] < errs[0]
+        assert np.array_equal(td.cpu().numpy(), d)
+
+
+@pytest.mark.parametrize("shape", [(32, 16, 32, 8, 3), (16, 32, 32, 8, 3), (16, 32, 16, 8, 0),
+                                   (8, 64, 8, 8, 0)])
+def test_mixed_f32_forced_strided_shapes(shape):
+    """Each strided kernel shape forced along y and z of a 40 x 30 x 500 grid
+    (partial 32-system groups): the wide kernel at 32 and 16 rows per lane
+    (the latter at 1024 threads), the register kernel at 16 and 8; one and
+    two refinements (two take the general refinement loop)."""
+    M, P, W, G, V = shape
+    rng = np.random.default_rng(M * 7 + V)
+    dims = [40, 30, 500]
+    n = int(np.prod(dims))
+    a, b, c, d = _f32_case(rng, n)
+    t = [dev(v) for v in (a, b, c, d)]
+    for sd in (1, 2):
+        N = dims[sd]
+        Pn = -(-N // M)
+        if V == 3:
+            Pn += Pn & 1
+        if Pn > P:
+            continue  # longer than this shape holds
+        xl = oracle.batch_ld(a, b, c, d, ndim=3, solvedim=sd, dims=dims)
+        with api.forced_hybrid(M, Pn, W, G, V):
+            for refine in (1, 2):
+                x = api.mixed_f32_strided_batch(*t, solvedim=sd, dims=dims, refinements=refine,
+                                                sync=True)
+                assert api.last_config()[1:4] == (M, Pn, W)
+                assert H.norm_rel_err(x.cpu().numpy(), xl) <= 1e-12, (sd, refine)
+
+
+def test_mixed_f32_padded_layout():
+    """Halo-padded layout: x gets the solution at the interior offsets and
+    keeps its halo."""
+    rng = np.random.default_rng(91)
+    dims = [30, 22, 14]
+    n = int(np.prod(dims))
+    a, b, c, d = _f32_case(rng, n)
+    arrs = [H.pad_halo(v, dims) for v in (a, b, c, d)]
+    off, pads = H.halo_offset(dims), [v + 2 for v in dims]
+    t = [dev(v) for v in arrs]
+    mask = H.pad_halo(np.ones(n, dtype=bool), dims)
+    for sd in range(3):
+        x0 = torch.full((arrs[0].size,), 7.0, dtype=torch.float64, device="cuda")
+        x = api.mixed_f32_strided_batch(*t, x0, solvedim=sd, dims=dims, pads=pads, offset=off,
+                                        refinements=1, sync=True).cpu().numpy()
+        want = oracle.batch_ld(a, b, c, d, ndim=3, solvedim=sd, dims=dims)
+        assert np.all(x[~mask] == 7.0)
+        H.assert_norm_close(x[mask], want, np.float64, tol=1e-12)
+
+
+def test_mixed_f32_rejects_bad_arguments():
+    rng = np.random.default_rng(92)
+    dims = [16, 8, 4]
+    a, b, c, d = (dev(v) for v in _f32_case(rng, int(np.prod(dims))))
+    with pytest.raises(TypeError):
+        api.mixed_f32_strided_batch(a.double(), b.double(), c.double(), d.double(), solvedim=0,
+                                    dims=dims)
+    with pytest.raises(Exception):  # refinements out of range
+        api.mixed_f32_strided_batch(a, b, c, d, solvedim=0, dims=dims, refinements=9, sync=True)
+    with api.forced_hybrid(16, 1, 32, 8, 3):  # wide kernel shape on contiguous systems
+        with pytest.raises(Exception):
+            api.mixed_f32_strided_batch(a, b, c, d, solvedim=0, dims=dims, sync=True)
+
+
 def test_mixed_lds_rejects_unfit_shapes():
     rng = np.random.default_rng(79)
     dims = [64, 600, 2]

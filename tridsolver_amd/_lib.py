@@ -65,6 +65,7 @@ SIGNATURES = {
     "tridHipForceHybridConfig": ([I, I, I, I, I], I),
     "tridHipStreamCopy": ([P, P, SZ], I),
     "tridDmtsvStridedBatchMixed": ([P, P, P, P, P, I, I, P, P, I], I),
+    "tridDmtsvStridedBatchMixedF32": ([P, P, P, P, P, P, I, I, P, P, I], I),
     "tridDmtsvStridedBatchAdjoint": ([P] * 10 + [I, I, P, P], I),
     "tridSmtsvStridedBatchAdjoint": ([P] * 10 + [I, I, P, P], I),
     "tridDmtsvRaggedBatch": ([P] * 6 + [L, L], I),

@@ -186,6 +186,29 @@ def mixed_strided_batch(a, b, c, d, *, solvedim, dims=None, pads=None, offset=0,
     return d
 
 
+def mixed_f32_strided_batch(a, b, c, d, x=None, *, solvedim, dims=None, pads=None, offset=0,
+                            refinements=1, sync=False, stream=None):
+    """FP64 solution of the system stored in FP32 (tridDmtsvStridedBatchMixedF32):
+    a, b, c, d float32, x float64 of the same shape (allocated when None);
+    returns x. d is not written."""
+    if d.dtype != torch.float32:
+        raise TypeError("the FP32-stored solve takes float32 operands")
+    x = torch.empty(d.shape, dtype=torch.float64, device=d.device) if x is None else x
+    if x.dtype != torch.float64 or x.numel() != d.numel() or x.device != d.device:
+        raise TypeError("x must be a float64 tensor of d's size on d's device")
+    dims = list(d.shape[::-1]) if dims is None else dims
+    pads = (list(d.shape[::-1]) if d.dim() == len(dims) else list(dims)) if pads is None else pads
+    _validate([a, b, c, d], dims, pads, offset)
+    _validate([x], dims, pads, offset)
+    use_torch_stream(stream)
+    params = make_params((0, 0, 0), 1 if sync else 0)
+    st = _lib.lib().tridDmtsvStridedBatchMixedF32(
+        C.byref(params), *[_ptr(t, offset) for t in (a, b, c, d, x)], len(dims), solvedim,
+        int3(dims), int3(pads), int(refinements))
+    check(st, "tridDmtsvStridedBatchMixedF32")
+    return x
+
+
 def adjoint_strided_batch(a, b, c, x, w, *, solvedim, dims=None, pads=None, offset=0,
                           lam=None, grad_a=None, grad_b=None, grad_c=None, sync=False,
                           stream=None):

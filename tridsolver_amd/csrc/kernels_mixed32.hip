@@ -1,0 +1,339 @@
+// kernels_mixed32.hip - FP64 solution of batched systems stored in FP32, by
+// FP32 sweeps plus FP64 iterative refinement, in one pass over HBM (no
+// counterpart in the reference).
+//
+// The system solved is the one whose coefficients and right-hand side are
+// the FP32 values, each exactly representable in FP64: A x = d with a, b, c
+// and d promoted. The factorisation and the sweeps run on the FP32 values as
+// they are; the residual d - A x is formed in FP64 from the promoted values
+// and x += dx is accumulated in FP64 (mixed_core.hpp). x goes to a separate
+// FP64 array of the operands' layout. The FP32 factorisation takes its
+// reciprocals from the hardware's approximate reciprocal (mixed_core.hpp,
+// FAST). HBM traffic per point: 16 B read (a, b, c, d) and 8 B written (x),
+// against the 40 B the FP64 solve of the same system moves when it is
+// stored in FP64.
+//
+// Two kernels:
+//   mixed32_kernel       every operand in registers; contiguous systems go
+//                        through an LDS tile per system (float operands in,
+//                        FP64 x out), strided ones take W x-neighbours.
+//   mixed32_wide_kernel  strided systems, 32 x-neighbours per workgroup, so
+//                        every row is one 128-byte segment (what keeps the
+//                        z solve's address translations per byte low); `a`
+//                        and `c` go straight from HBM into LDS tiles
+//                        [row][system] by global->LDS loads (one dword per
+//                        lane: one FP32 element), b and d into registers.
+
+#include <algorithm>
+#include <atomic>
+
+#include "../../include/tridsolver_hip.h"
+#include "hybrid_core.hpp"
+#include "mixed_core.hpp"
+#include "trid_internal.h"
+
+namespace trid {
+namespace {
+
+constexpr int M32_THREADS = 512;
+constexpr size_t M32_LDS_MAX = 160 * 1024;
+
+template <int M, bool CONTIG, bool ONE>
+__global__ __launch_bounds__(M32_THREADS) void mixed32_kernel(
+    Geom g, int P, int W, int G, int refine, const float *__restrict__ a,
+    const float *__restrict__ b, const float *__restrict__ c, const float *__restrict__ d,
+    double *__restrict__ x) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const hyb::Lane L = hyb::lane_of<CONTIG>(g, P, W, G);
+  // contiguous: one P*(M+1)-double tile per system; the FP32 operands use
+  // the first half of it, X (FP64) all of it
+  const size_t tile_bytes = CONTIG ? sizeof(double) * (size_t)W * P * (M + 1) : 0;
+  double *dtile = reinterpret_cast<double *>(smem_raw) + (size_t)L.w * P * (M + 1);
+  float *ftile = reinterpret_cast<float *>(dtile);
+  hyb::Ring ring{smem_raw + tile_bytes, (int)(2 * sizeof(double) * L.T), 0};
+  float A[M], B[M], Cc[M], D[M];
+  hyb::fetch_abcd<M, CONTIG>(L, a, b, c, d, A, B, Cc, D);
+  hyb::to_chunk<M, CONTIG>(L, A, ftile);
+  hyb::to_chunk<M, CONTIG>(L, B, ftile);
+  hyb::to_chunk<M, CONTIG>(L, Cc, ftile);
+  hyb::to_chunk<M, CONTIG>(L, D, ftile);
+  double X[M];
+  mixed_chunk_acc<M, MIXED_MAX_STEPS, false, ONE, true>(
+      L, ring, [&](int r) { return A[r]; }, B, [&](int r) { return Cc[r]; }, D, refine, X);
+  hyb::to_fetch<M, CONTIG>(L, X, dtile);
+  hyb::store<M, CONTIG>(L, x, X);
+}
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+constexpr int M32_WIDE_W = 32;  // FP32 x-neighbours per workgroup: 128-byte rows
+
+// Threads per workgroup of the wide kernel: M = 32 rows per lane in 512
+// threads (two waves per SIMD), M = 16 in 1024 (four).
+template <int M>
+struct M32Wide {
+  static constexpr int threads = M >= 32 ? 512 : 1024;
+  static constexpr int steps = M >= 32 ? 4 : 5;  // PCR steps: P <= threads / 32
+};
+
+template <int M, bool ONE>
+__global__ __launch_bounds__(M32Wide<M>::threads) void mixed32_wide_kernel(
+    Geom g, int P, int G, int refine, const float *__restrict__ a, const float *__restrict__ b,
+    const float *__restrict__ c, const float *__restrict__ d, double *__restrict__ x) {
+  constexpr int W = M32_WIDE_W;
+  constexpr int SEG = W * (int)sizeof(float);
+  constexpr int K = 256 / SEG;  // rows per wave-wide global->LDS load
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const hyb::Lane L = hyb::lane_of<false>(g, P, W, G);
+  const int T = L.T;
+  const int64_t R = (int64_t)P * M;  // rows per tile (>= N)
+  float *tA = reinterpret_cast<float *>(smem_raw);
+  float *tC = tA + (size_t)R * W;
+  hyb::Ring ring{smem_raw + 2 * sizeof(float) * (size_t)R * W, (int)(2 * sizeof(double) * T),
+                 0};
+
+  // (0) a rows 1..N-1 and c rows 0..N-2 -> tiles [row][system]. One wave
+  // instruction writes 256 contiguous LDS bytes, lane l its bytes 4l..4l+3:
+  // K rows of SEG bytes; lane l serves row (first row + 4l / SEG) and system
+  // (4l % SEG) / 4. Systems beyond n0 are masked off; the slots no load
+  // writes (a row 0, c row N-1, rows >= N, systems beyond n0) are cleared by
+  // their own lanes below.
+  {
+    const int lane = L.t & 63, wv = L.t >> 6, nw = T >> 6;
+    const int sub = (lane * 4) / SEG, byte = (lane * 4) % SEG;
+    const int64_t first = L.i0 - L.w;
+    const bool sys_ok = first + byte / (int)sizeof(float) < g.n0;
+    const int64_t row_bytes = L.stride * (int64_t)sizeof(float);
+    const int64_t o = (first + L.i1 * g.st1) * (int64_t)sizeof(float) + byte + sub * row_bytes;
+    const char *srcA = reinterpret_cast<const char *>(a) + o;
+    const char *srcC = reinterpret_cast<const char *>(c) + o;
+    if (sys_ok) {
+      for (int64_t row = (int64_t)K * wv; row < L.N; row += (int64_t)K * nw) {
+        const int64_t rr = row + sub;
+        if (rr >= 1 && rr < L.N)
+          __builtin_amdgcn_global_load_lds(srcA + row * row_bytes, (lds_void *)(tA + row * W), 4,
+                                           0, 0);
+        if (rr < L.N - 1)
+          __builtin_amdgcn_global_load_lds(srcC + row * row_bytes, (lds_void *)(tC + row * W), 4,
+                                           0, 0);
+      }
+    }
+  }
+
+  // b, d of the lane's rows into registers (identity rows past N). Element
+  // offsets are 32-bit (the launch checks that x's extent is below 4 GiB,
+  // so the FP32 arrays' is below 2 GiB).
+  const int64_t row0 = (int64_t)L.p * M;
+  const uint32_t oe = (uint32_t)(L.base + row0 * L.stride);
+  const uint32_t se = (uint32_t)L.stride;
+  const bool full = L.active && row0 + M <= L.N;
+  float B[M], D[M];
+  if (full) {
+#pragma unroll
+    for (int r = 0; r < M; ++r) {
+      const uint32_t e = oe + (uint32_t)r * se;
+      B[r] = b[e];
+      D[r] = d[e];
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < M; ++r) {
+      const bool ok = L.active && row0 + r < L.N;
+      const uint32_t e = oe + (uint32_t)r * se;
+      B[r] = ok ? b[e] : 1.0f;
+      D[r] = ok ? d[e] : 0.0f;
+    }
+  }
+  float *sa = tA + row0 * W + L.w;  // sa[r * W]: this lane's row r
+  float *sc = tC + row0 * W + L.w;
+#pragma unroll
+  for (int r = 0; r < M; ++r) {
+    const int64_t row = row0 + r;
+    if (!L.active || row == 0 || row >= L.N) sa[r * W] = 0.0f;
+    if (!L.active || row >= L.N - 1) sc[r * W] = 0.0f;
+  }
+  // the tiles are written by other waves' loads: wait for this wave's, then
+  // for every wave
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+
+  double X[M];
+  mixed_chunk_acc<M, M32Wide<M>::steps, false, ONE, true>(
+      L, ring, [&](int r) { return sa[r * W]; }, B, [&](int r) { return sc[r * W]; }, D, refine,
+      X);
+  if (!L.active) return;
+  // store offsets recomputed (the empty asm hides that they equal the load
+  // offsets, which would otherwise stay live across the solve)
+  uint32_t oes = oe;
+  asm volatile("" : "+v"(oes));
+  const int rows = full ? M : (int)std::min<int64_t>(M, std::max<int64_t>(0, L.N - row0));
+#pragma unroll
+  for (int r = 0; r < M; ++r)
+    if (r < rows) x[oes + (uint32_t)r * se] = X[r];
+}
+
+size_t reg_lds_bytes(const Geom &g, const HybridCfg &c) {
+  const size_t T = (size_t)c.W * c.P;
+  size_t lds = hyb::ring_bytes<double>((int)T, 2);
+  if (g.stride == 1) lds += sizeof(double) * T * (c.M + 1);
+  return lds;
+}
+
+size_t wide_lds_bytes(const HybridCfg &c) {
+  const size_t T = (size_t)c.W * c.P;
+  return 2 * sizeof(float) * (size_t)c.P * c.M * c.W + hyb::ring_bytes<double>((int)T, 2);
+}
+
+bool wide_valid(const Geom &g, const HybridCfg &c) {
+  if (g.stride == 1 || g.st0 != 1 || c.W != M32_WIDE_W) return false;
+  const int maxT = c.M == 32 ? M32Wide<32>::threads : c.M == 16 ? M32Wide<16>::threads : 0;
+  const int steps = c.M == 32 ? M32Wide<32>::steps : M32Wide<16>::steps;
+  if (maxT == 0 || c.P < 1 || c.G < 1) return false;
+  const int64_t T = (int64_t)c.W * c.P;
+  if ((int64_t)c.P * c.M < g.N || c.P > (1 << steps) || T > maxT || T % 64 != 0) return false;
+  // 32-bit element offsets, and x's byte extent below 4 GiB
+  const int64_t nb0 = (g.n0 + c.W - 1) / c.W * c.W;
+  const int64_t last = (nb0 - 1) + (g.n1 - 1) * g.st1 + ((int64_t)c.P * c.M - 1) * g.stride;
+  if ((last + 1) * (int64_t)sizeof(double) > (int64_t)0xFFFFFFFFll) return false;
+  return wide_lds_bytes(c) <= M32_LDS_MAX;
+}
+
+bool reg_valid(const Geom &g, const HybridCfg &c) {
+  if (c.M != 8 && c.M != 16) return false;
+  if (c.P < 1 || c.W < 1 || c.G < 1) return false;
+  if ((int64_t)c.P * c.M < g.N || c.P > (1 << MIXED_MAX_STEPS)) return false;
+  if ((int64_t)c.W * c.P > M32_THREADS) return false;
+  return reg_lds_bytes(g, c) <= M32_LDS_MAX;
+}
+
+template <int M, bool CONTIG, bool ONE>
+hipError_t launch_reg_one(const Geom &g, const HybridCfg &cfg, int refine, const float *a,
+                          const float *b, const float *c, const float *d, double *x,
+                          hipStream_t st) {
+  const int T = cfg.W * cfg.P;
+  const size_t lds = reg_lds_bytes(g, cfg);
+  auto kern = mixed32_kernel<M, CONTIG, ONE>;
+  static LdsGrant lds_granted;
+  {
+    hipError_t e = ensure_lds(reinterpret_cast<const void *>(kern), lds, lds_granted);
+    if (e != hipSuccess) return e;
+  }
+  const int64_t blocks = (g.n0 + cfg.W - 1) / cfg.W * g.n1;
+  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(T), lds, st, g, cfg.P, cfg.W, cfg.G,
+                     refine, a, b, c, d, x);
+  return hipGetLastError();
+}
+
+// At most one refinement runs the kernels built for it (fewer registers).
+template <int M, bool CONTIG>
+hipError_t launch_reg(const Geom &g, const HybridCfg &cfg, int refine, const float *a,
+                      const float *b, const float *c, const float *d, double *x,
+                      hipStream_t st) {
+  return refine <= 1 ? launch_reg_one<M, CONTIG, true>(g, cfg, refine, a, b, c, d, x, st)
+                     : launch_reg_one<M, CONTIG, false>(g, cfg, refine, a, b, c, d, x, st);
+}
+
+template <int M, bool ONE>
+hipError_t launch_wide32_one(const Geom &g, const HybridCfg &cfg, int refine, const float *a,
+                             const float *b, const float *c, const float *d, double *x,
+                             hipStream_t st) {
+  const int T = cfg.W * cfg.P;
+  const size_t lds = wide_lds_bytes(cfg);
+  auto kern = mixed32_wide_kernel<M, ONE>;
+  static LdsGrant lds_granted;
+  {
+    hipError_t e = ensure_lds(reinterpret_cast<const void *>(kern), lds, lds_granted);
+    if (e != hipSuccess) return e;
+  }
+  const int64_t blocks = (g.n0 + cfg.W - 1) / cfg.W * g.n1;
+  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(T), lds, st, g, cfg.P, cfg.G, refine, a,
+                     b, c, d, x);
+  return hipGetLastError();
+}
+
+template <int M>
+hipError_t launch_wide32(const Geom &g, const HybridCfg &cfg, int refine, const float *a,
+                         const float *b, const float *c, const float *d, double *x,
+                         hipStream_t st) {
+  return refine <= 1 ? launch_wide32_one<M, true>(g, cfg, refine, a, b, c, d, x, st)
+                     : launch_wide32_one<M, false>(g, cfg, refine, a, b, c, d, x, st);
+}
+
+}  // namespace
+
+// V = 0: mixed32_kernel (M = 8 or 16, W * P <= 512); V = 3: the wide
+// strided kernel (W = 32, M = 16 or 32).
+bool mixed32_cfg_valid(const Geom &g, const HybridCfg &c) {
+  if (c.V == 3) return wide_valid(g, c);
+  if (c.V != 0) return false;
+  return reg_valid(g, c);
+}
+
+// Default shapes, from a sweep at 512^3 (one refinement): contiguous
+// systems 16 rows per lane from 256 rows on (else 8), in workgroups of up to
+// 256 threads (x: 0.69 ms, against 0.74 at 8 rows per lane and 0.85 at 512
+// threads); strided systems the wide kernel at 32 rows per lane with the
+// workgroups dealt to the XCDs in 32 groups (y 0.68, z 0.81 ms; 16 rows per
+// lane in 1024 threads spill a few registers and run 0.82 / 1.15 ms), else
+// 16 x-neighbours in registers.
+// More than one refinement needs x in FP64 and d kept through the loop: the
+// shapes above spill there, so those calls take 8 rows per lane in registers.
+bool mixed32_default_cfg(const Geom &g, int refine, HybridCfg *cfg) {
+  HybridCfg c;
+  c.G = 8;
+  if (refine > 1) {
+    c.M = 8;
+    c.V = 0;
+    c.P = (int)((g.N + c.M - 1) / c.M);
+    c.W = g.stride == 1 ? (int)std::max<int64_t>(1, 256 / c.P) : 8;
+    while (c.W > 1 && (int64_t)c.W * c.P > M32_THREADS) c.W /= 2;
+    if (reg_valid(g, c)) { *cfg = c; return true; }
+  }
+  if (g.stride != 1) {
+    for (int M : {32, 16}) {
+      c.M = M;
+      c.V = 3;
+      c.G = 32;
+      c.W = M32_WIDE_W;
+      c.P = (int)((g.N + M - 1) / M);
+      c.P += c.P & 1;  // whole waves: 32 * P threads
+      if (wide_valid(g, c)) { *cfg = c; return true; }
+    }
+  }
+  c.G = 8;
+  for (int M : {g.N >= 256 ? 16 : 8, 16}) {
+    c.M = M;
+    c.V = 0;
+    c.P = (int)((g.N + M - 1) / M);
+    if (g.stride == 1) {
+      c.W = (int)std::max<int64_t>(1, 256 / c.P);
+    } else {
+      c.W = 16;
+      while (c.W > 1 && (int64_t)c.W * c.P > M32_THREADS) c.W /= 2;
+    }
+    if (reg_valid(g, c)) { *cfg = c; return true; }
+  }
+  return false;
+}
+
+hipError_t launch_mixed32(const Geom &g, const HybridCfg &cfg, int refine, const float *a,
+                          const float *b, const float *c, const float *d, double *x,
+                          hipStream_t st) {
+  const bool contig = g.stride == 1;
+  if (cfg.V == 3) {
+    switch (cfg.M) {
+      case 16: return launch_wide32<16>(g, cfg, refine, a, b, c, d, x, st);
+      case 32: return launch_wide32<32>(g, cfg, refine, a, b, c, d, x, st);
+      default: return hipErrorInvalidValue;
+    }
+  }
+  switch (cfg.M) {
+    case 8: return contig ? launch_reg<8, true>(g, cfg, refine, a, b, c, d, x, st)
+                          : launch_reg<8, false>(g, cfg, refine, a, b, c, d, x, st);
+    case 16: return contig ? launch_reg<16, true>(g, cfg, refine, a, b, c, d, x, st)
+                           : launch_reg<16, false>(g, cfg, refine, a, b, c, d, x, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace trid

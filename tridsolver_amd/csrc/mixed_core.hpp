@@ -29,30 +29,50 @@ namespace trid {
 
 constexpr int MIXED_MAX_STEPS = 9;  // PCR steps for P <= 512
 
-// Aof(r), Cof(r): the chunk's FP64 sub- and super-diagonal at row r (0 where
-// masked). The ring needs slots of 2 doubles per thread.
-template <int M, int STEPS, typename GetA, typename GetC>
+// Aof(r), Cof(r): the chunk's sub- and super-diagonal at row r (0 where
+// masked). The operands are FP64 (TB = double), or FP32 values that define
+// the system exactly (TB = float: the FP64 residual promotes them). The ring
+// needs slots of 2 doubles per thread. With STORE_INV false the pivots
+// 1/beta_i are recomputed by every right-hand-side solve instead of being
+// kept in registers (same expression, same values). ONE is for callers that
+// never ask for more than one refinement: x0 stays in FP32 registers (its
+// FP64 value is exact), and x = x0 + dx is formed in FP64 only at the end --
+// the same values as the general loop, with M fewer registers live. FAST
+// takes the FP32 reciprocals of the factorisation from the hardware's
+// approximate reciprocal (v_rcp_f32, about 1 ulp) instead of a correctly
+// rounded division: the factorisation only has to be close to A's, the FP64
+// residual and the refinement make up the difference.
+template <int M, int STEPS, bool STORE_INV = true, bool ONE = false, bool FAST = false,
+          typename TB, typename GetA, typename GetC>
 __device__ __forceinline__ void mixed_chunk_acc(const hyb::Lane &L, hyb::Ring &ring, GetA Aof,
-                                                const double (&B)[M], GetC Cof,
-                                                const double (&D)[M], int refine,
-                                                double (&X)[M]) {
+                                                const TB (&B)[M], GetC Cof, const TB (&D)[M],
+                                                int refine, double (&X)[M]) {
   const int T = L.T, t = L.t, p = L.p, ps = L.pstep, P = L.P;
 
   // ---- FP32 factorisation ------------------------------------------------
   // Per row: inv = 1/beta, ga and the forward spike al. The backward sweep
   // is kept only for the chunk's first row (f_a, f_g); the substitution runs
   // from the chunk's end on the forward values, so ap and gp are not stored.
-  float inv[M], ga[M], al[M];
-  inv[0] = 1.0f / (float)B[0];
-  al[0] = (float)Aof(0) * inv[0];
-  ga[0] = (float)Cof(0) * inv[0];
+  float inv[STORE_INV ? M : 1], ga[M], al[M];
+  auto rcp = [](float v) -> float { return FAST ? __builtin_amdgcn_rcpf(v) : 1.0f / v; };
+  auto pivot = [&](int r) -> float {  // 1/beta_r, given ga[r-1]
+    return r == 0 ? rcp((float)B[0]) : rcp((float)B[r] - (float)Aof(r) * ga[r - 1]);
+  };
+  {
+    const float i0 = pivot(0);
+    if (STORE_INV) inv[0] = i0;
+    al[0] = (float)Aof(0) * i0;
+    ga[0] = (float)Cof(0) * i0;
+  }
 #pragma unroll
   for (int r = 1; r < M; ++r) {
     const float ar = (float)Aof(r);
-    inv[r] = 1.0f / ((float)B[r] - ar * ga[r - 1]);
-    al[r] = -ar * al[r - 1] * inv[r];
-    ga[r] = (float)Cof(r) * inv[r];
+    const float ir = pivot(r);
+    if (STORE_INV) inv[STORE_INV ? r : 0] = ir;
+    al[r] = -ar * al[r - 1] * ir;
+    ga[r] = (float)Cof(r) * ir;
   }
+  auto inv_of = [&](int r) -> float { return STORE_INV ? inv[STORE_INV ? r : 0] : pivot(r); };
   const float al_e = al[M - 1], ga_e = ga[M - 1];
   float f_a = 0.0f, f_g = -1.0f;
   if (M >= 2) {
@@ -71,7 +91,7 @@ __device__ __forceinline__ void mixed_chunk_acc(const hyb::Lane &L, hyb::Ring &r
     __syncthreads();
     if (p + 1 < P) { na = q[t + ps]; ng = q[T + t + ps]; }
   }
-  const float invB = 1.0f / (1.0f - ga_e * na);
+  const float invB = rcp(1.0f - ga_e * na);
   float RA = al_e * invB, RC = -ga_e * ng * invB;
   float sA[STEPS], sC[STEPS], sR[STEPS];
 #pragma unroll
@@ -85,7 +105,7 @@ __device__ __forceinline__ void mixed_chunk_acc(const hyb::Lane &L, hyb::Ring &r
       float Am = 0, Cm = 0, Ap = 0, Cp = 0;
       if (p - h >= 0) { Am = q[t - h * ps]; Cm = q[T + t - h * ps]; }
       if (p + h < P) { Ap = q[t + h * ps]; Cp = q[T + t + h * ps]; }
-      const float rs = 1.0f / (1.0f - RA * Cm - RC * Ap);
+      const float rs = rcp(1.0f - RA * Cm - RC * Ap);
       sR[s] = rs;
       RA = -rs * RA * Am;
       RC = -rs * RC * Cp;
@@ -94,9 +114,9 @@ __device__ __forceinline__ void mixed_chunk_acc(const hyb::Lane &L, hyb::Ring &r
 
   // ---- FP32 right-hand-side solve with the stored factorisation ----------
   auto solve_rhs = [&](float (&v)[M]) {
-    v[0] = v[0] * inv[0];
+    v[0] = v[0] * inv_of(0);
 #pragma unroll
-    for (int r = 1; r < M; ++r) v[r] = (v[r] - (float)Aof(r) * v[r - 1]) * inv[r];
+    for (int r = 1; r < M; ++r) v[r] = (v[r] - (float)Aof(r) * v[r - 1]) * inv_of(r);
     const float de_e = v[M - 1];
     float f_d = 0.0f;
     if (M >= 2) {
@@ -137,6 +157,38 @@ __device__ __forceinline__ void mixed_chunk_acc(const hyb::Lane &L, hyb::Ring &r
     for (int r = M - 2; r >= 0; --r) v[r] = v[r] - al[r] * ym - ga[r] * v[r + 1];
   };
 
+  if constexpr (ONE) {
+    float v[M];
+#pragma unroll
+    for (int r = 0; r < M; ++r) v[r] = (float)D[r];
+    solve_rhs(v);
+    if (refine > 0) {
+      float xprev = 0, xnext = 0;  // neighbour chunks' boundary values of x0
+      {
+        float *q = ring.next<float>();
+        q[t] = v[0]; q[T + t] = v[M - 1];
+        __syncthreads();
+        if (p > 0) xprev = q[T + t - ps];
+        if (p + 1 < P) xnext = q[t + ps];
+      }
+      float w[M];
+#pragma unroll
+      for (int r = 0; r < M; ++r) {
+        const double xl = r > 0 ? v[r - 1] : xprev;
+        const double xr = r < M - 1 ? v[r + 1] : xnext;
+        const double res = (double)D[r] - ((double)B[r] * (double)v[r] + (double)Aof(r) * xl +
+                                           (double)Cof(r) * xr);
+        w[r] = (float)res;
+      }
+      solve_rhs(w);
+#pragma unroll
+      for (int r = 0; r < M; ++r) X[r] = (double)v[r] + (double)w[r];
+    } else {
+#pragma unroll
+      for (int r = 0; r < M; ++r) X[r] = (double)v[r];
+    }
+    return;
+  }
   {
     float v[M];
 #pragma unroll
@@ -160,7 +212,8 @@ __device__ __forceinline__ void mixed_chunk_acc(const hyb::Lane &L, hyb::Ring &r
     for (int r = 0; r < M; ++r) {
       const double xl = r > 0 ? X[r - 1] : xprev;
       const double xr = r < M - 1 ? X[r + 1] : xnext;
-      const double res = D[r] - (B[r] * X[r] + Aof(r) * xl + Cof(r) * xr);
+      const double res = (double)D[r] - ((double)B[r] * X[r] + (double)Aof(r) * xl +
+                                         (double)Cof(r) * xr);
       v[r] = (float)res;
     }
     solve_rhs(v);

@@ -136,6 +136,33 @@ EXTERN_C tridStatus_t tridDmtsvStridedBatchMixed(const TridParams *ctx, const do
   return finish(ctx, st, launch_mixed(g, cfg, refinements, a, b, c, d, st));
 }
 
+EXTERN_C tridStatus_t tridDmtsvStridedBatchMixedF32(const TridParams *ctx, const float *a,
+                                                    const float *b, const float *c,
+                                                    const float *d, double *x, int ndim,
+                                                    int solvedim, const int *dims,
+                                                    const int *pads, int refinements) {
+  Geom g;
+  bool empty = false;
+  tridStatus_t s = make_geom(ndim, solvedim, dims, pads, &g, &empty);
+  if (s != TRID_STATUS_SUCCESS) return s;
+  if (!a || !b || !c || !d || !x || refinements < 0 || refinements > 8)
+    return TRID_STATUS_INVALID_VALUE;
+  int dev = 0;
+  if ((s = check_device(&dev)) != TRID_STATUS_SUCCESS) return s;
+  hipStream_t st = current_stream();
+  if (empty) return finish(ctx, st, hipSuccess);
+  HybridCfg cfg;
+  const HybridCfg &forced = forced_config();
+  if (forced.M != 0) {
+    cfg = forced;
+    if (!mixed32_cfg_valid(g, cfg)) return TRID_STATUS_INVALID_VALUE;
+  } else if (!mixed32_default_cfg(g, refinements, &cfg)) {
+    return TRID_STATUS_INVALID_VALUE;
+  }
+  record_config(10, &cfg);
+  return finish(ctx, st, launch_mixed32(g, cfg, refinements, a, b, c, d, x, st));
+}
+
 EXTERN_C tridStatus_t tridDmtsvStridedBatchAdjoint(const TridParams *ctx, const double *a,
                                                    const double *b, const double *c,
                                                    const double *x, const double *w,

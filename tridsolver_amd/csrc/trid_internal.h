@@ -37,7 +37,8 @@ struct Geom {
 // sweep); 1 and 2 the streamed strided variants (rows loaded a few ahead of
 // the sweep; kernels_stream.hip); 3 the strided variant with `a` staged in
 // LDS (kernels_wide.hip). For the mixed-precision solve, 3 is the M = 16
-// kernel with the FP64 a and c operands in LDS (kernels_mixed.hip).
+// kernel with the FP64 a and c operands in LDS (kernels_mixed.hip); for the
+// solve of FP32-stored systems, the wide strided kernel (kernels_mixed32.hip).
 struct HybridCfg {
   int M = 0;
   int P = 0;
@@ -93,6 +94,13 @@ bool mixed_cfg_valid(const Geom &g, const HybridCfg &c);
 bool mixed_default_cfg(const Geom &g, HybridCfg *cfg);
 hipError_t launch_mixed(const Geom &g, const HybridCfg &cfg, int refine, const double *a,
                         const double *b, const double *c, double *d, hipStream_t st);
+
+// ---- FP64 solution of systems stored in FP32 (kernels_mixed32.hip) -------
+bool mixed32_cfg_valid(const Geom &g, const HybridCfg &c);
+bool mixed32_default_cfg(const Geom &g, int refine, HybridCfg *cfg);
+hipError_t launch_mixed32(const Geom &g, const HybridCfg &cfg, int refine, const float *a,
+                          const float *b, const float *c, const float *d, double *x,
+                          hipStream_t st);
 
 // ---- adjoint (kernels_adjoint.hip) ----------------------------------------
 template <typename REAL>
