@@ -118,8 +118,7 @@ EXTERN_C tridStatus_t tridDmtsvStridedBatchMixed(const TridParams *ctx, const do
  * residual d - A x and x += dx in FP64, `refinements` (0..8) times; 1 reaches
  * about 1e-13 of the FP64 solution on diagonally dominant systems. One pass
  * over HBM: 16 B read and 8 B written per point, against 40 B for the FP64
- * solve of the same system stored in FP64. Systems up to 8192 rows along a
- * contiguous dimension and 4096 along a strided one. */
+ * solve of the same system stored in FP64. Systems up to 8192 rows. */
 EXTERN_C tridStatus_t tridDmtsvStridedBatchMixedF32(const TridParams *ctx, const float *a,
                                                     const float *b, const float *c,
                                                     const float *d, double *x, int ndim,

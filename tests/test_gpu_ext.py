@@ -166,7 +166,7 @@ def _f32_case(rng, n):
 
 
 @pytest.mark.parametrize("dims", [[64, 48, 40], [37, 21, 19], [512, 3, 2], [9, 700, 2],
-                                  [1000], [8192, 3], [3, 4096], [96, 80, 512]])
+                                  [1000], [8192, 3], [3, 4096], [2, 8192], [96, 80, 512]])
 def test_mixed_f32_vs_long_double_oracle(dims):
     """The system is the FP32 values promoted to FP64: 0 refinements is
     FP32-accurate, 1 and 2 are within 1e-12 of the long-double oracle's
@@ -188,13 +188,14 @@ def test_mixed_f32_vs_long_double_oracle(dims):
         assert np.array_equal(td.cpu().numpy(), d)
 
 
-@pytest.mark.parametrize("shape", [(32, 16, 32, 8, 3), (16, 32, 32, 8, 3), (16, 32, 16, 8, 0),
-                                   (8, 64, 8, 8, 0)])
+@pytest.mark.parametrize("shape", [(32, 16, 32, 8, 3), (16, 32, 32, 8, 3), (32, 16, 16, 8, 3),
+                                   (16, 32, 16, 8, 0), (8, 64, 8, 8, 0)])
 def test_mixed_f32_forced_strided_shapes(shape):
     """Each strided kernel shape forced along y and z of a 40 x 30 x 500 grid
-    (partial 32-system groups): the wide kernel at 32 and 16 rows per lane
-    (the latter at 1024 threads), the register kernel at 16 and 8; one and
-    two refinements (two take the general refinement loop)."""
+    (partial system groups): the wide kernel at 32 and 16 rows per lane with
+    128-byte rows (the latter at 1024 threads) and at 32 rows per lane with
+    64-byte rows, the register kernel at 16 and 8; one and two refinements
+    (two take the general refinement loop)."""
     M, P, W, G, V = shape
     rng = np.random.default_rng(M * 7 + V)
     dims = [40, 30, 500]
@@ -204,8 +205,8 @@ def test_mixed_f32_forced_strided_shapes(shape):
     for sd in (1, 2):
         N = dims[sd]
         Pn = -(-N // M)
-        if V == 3:
-            Pn += Pn & 1
+        if V == 3:  # whole waves: W * P a multiple of 64
+            Pn = -(-Pn // (64 // W)) * (64 // W)
         if Pn > P:
             continue  # longer than this shape holds
         xl = oracle.batch_ld(a, b, c, d, ndim=3, solvedim=sd, dims=dims)

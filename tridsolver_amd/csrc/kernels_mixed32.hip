@@ -19,7 +19,8 @@
 //                        FP64 x out), strided ones take W x-neighbours.
 //   mixed32_wide_kernel  strided systems, 32 x-neighbours per workgroup, so
 //                        every row is one 128-byte segment (what keeps the
-//                        z solve's address translations per byte low); `a`
+//                        z solve's address translations per byte low), or 16
+//                        (64-byte rows, two workgroups per CU); `a`
 //                        and `c` go straight from HBM into LDS tiles
 //                        [row][system] by global->LDS loads (one dword per
 //                        lane: one FP32 element), b and d into registers.
@@ -66,21 +67,20 @@ __global__ __launch_bounds__(M32_THREADS) void mixed32_kernel(
 
 typedef __attribute__((address_space(3))) void lds_void;
 
-constexpr int M32_WIDE_W = 32;  // FP32 x-neighbours per workgroup: 128-byte rows
-
-// Threads per workgroup of the wide kernel: M = 32 rows per lane in 512
-// threads (two waves per SIMD), M = 16 in 1024 (four).
-template <int M>
+// Threads per workgroup of the wide kernel with W FP32 x-neighbours (W = 32:
+// 128-byte rows, W = 16: 64-byte rows): M = 32 rows per lane at 16 lanes
+// per system (two waves per SIMD; W = 16 fits two workgroups per CU), M = 16
+// at 32 lanes (W = 32: 1024 threads, four waves per SIMD).
+template <int M, int W>
 struct M32Wide {
-  static constexpr int threads = M >= 32 ? 512 : 1024;
-  static constexpr int steps = M >= 32 ? 4 : 5;  // PCR steps: P <= threads / 32
+  static constexpr int threads = (M >= 32 ? 16 : 32) * W;
+  static constexpr int steps = M >= 32 ? 4 : 5;  // PCR steps: P <= threads / W
 };
 
-template <int M, bool ONE>
-__global__ __launch_bounds__(M32Wide<M>::threads) void mixed32_wide_kernel(
+template <int M, int W, bool ONE>
+__global__ __launch_bounds__((M32Wide<M, W>::threads)) void mixed32_wide_kernel(
     Geom g, int P, int G, int refine, const float *__restrict__ a, const float *__restrict__ b,
     const float *__restrict__ c, const float *__restrict__ d, double *__restrict__ x) {
-  constexpr int W = M32_WIDE_W;
   constexpr int SEG = W * (int)sizeof(float);
   constexpr int K = 256 / SEG;  // rows per wave-wide global->LDS load
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -158,7 +158,7 @@ __global__ __launch_bounds__(M32Wide<M>::threads) void mixed32_wide_kernel(
   __syncthreads();
 
   double X[M];
-  mixed_chunk_acc<M, M32Wide<M>::steps, false, ONE, true>(
+  mixed_chunk_acc<M, M32Wide<M, W>::steps, false, ONE, true>(
       L, ring, [&](int r) { return sa[r * W]; }, B, [&](int r) { return sc[r * W]; }, D, refine,
       X);
   if (!L.active) return;
@@ -185,9 +185,11 @@ size_t wide_lds_bytes(const HybridCfg &c) {
 }
 
 bool wide_valid(const Geom &g, const HybridCfg &c) {
-  if (g.stride == 1 || g.st0 != 1 || c.W != M32_WIDE_W) return false;
-  const int maxT = c.M == 32 ? M32Wide<32>::threads : c.M == 16 ? M32Wide<16>::threads : 0;
-  const int steps = c.M == 32 ? M32Wide<32>::steps : M32Wide<16>::steps;
+  if (g.stride == 1 || g.st0 != 1) return false;
+  int maxT = 0, steps = 0;
+  if (c.W == 32 && c.M == 32) { maxT = M32Wide<32, 32>::threads; steps = M32Wide<32, 32>::steps; }
+  if (c.W == 32 && c.M == 16) { maxT = M32Wide<16, 32>::threads; steps = M32Wide<16, 32>::steps; }
+  if (c.W == 16 && c.M == 32) { maxT = M32Wide<32, 16>::threads; steps = M32Wide<32, 16>::steps; }
   if (maxT == 0 || c.P < 1 || c.G < 1) return false;
   const int64_t T = (int64_t)c.W * c.P;
   if ((int64_t)c.P * c.M < g.N || c.P > (1 << steps) || T > maxT || T % 64 != 0) return false;
@@ -233,13 +235,13 @@ hipError_t launch_reg(const Geom &g, const HybridCfg &cfg, int refine, const flo
                      : launch_reg_one<M, CONTIG, false>(g, cfg, refine, a, b, c, d, x, st);
 }
 
-template <int M, bool ONE>
+template <int M, int W, bool ONE>
 hipError_t launch_wide32_one(const Geom &g, const HybridCfg &cfg, int refine, const float *a,
                              const float *b, const float *c, const float *d, double *x,
                              hipStream_t st) {
   const int T = cfg.W * cfg.P;
   const size_t lds = wide_lds_bytes(cfg);
-  auto kern = mixed32_wide_kernel<M, ONE>;
+  auto kern = mixed32_wide_kernel<M, W, ONE>;
   static LdsGrant lds_granted;
   {
     hipError_t e = ensure_lds(reinterpret_cast<const void *>(kern), lds, lds_granted);
@@ -251,18 +253,18 @@ hipError_t launch_wide32_one(const Geom &g, const HybridCfg &cfg, int refine, co
   return hipGetLastError();
 }
 
-template <int M>
+template <int M, int W>
 hipError_t launch_wide32(const Geom &g, const HybridCfg &cfg, int refine, const float *a,
                          const float *b, const float *c, const float *d, double *x,
                          hipStream_t st) {
-  return refine <= 1 ? launch_wide32_one<M, true>(g, cfg, refine, a, b, c, d, x, st)
-                     : launch_wide32_one<M, false>(g, cfg, refine, a, b, c, d, x, st);
+  return refine <= 1 ? launch_wide32_one<M, W, true>(g, cfg, refine, a, b, c, d, x, st)
+                     : launch_wide32_one<M, W, false>(g, cfg, refine, a, b, c, d, x, st);
 }
 
 }  // namespace
 
 // V = 0: mixed32_kernel (M = 8 or 16, W * P <= 512); V = 3: the wide
-// strided kernel (W = 32, M = 16 or 32).
+// strided kernel (W = 32 with M = 16 or 32; W = 16 with M = 32).
 bool mixed32_cfg_valid(const Geom &g, const HybridCfg &c) {
   if (c.V == 3) return wide_valid(g, c);
   if (c.V != 0) return false;
@@ -294,7 +296,7 @@ bool mixed32_default_cfg(const Geom &g, int refine, HybridCfg *cfg) {
       c.M = M;
       c.V = 3;
       c.G = 32;
-      c.W = M32_WIDE_W;
+      c.W = 32;
       c.P = (int)((g.N + M - 1) / M);
       c.P += c.P & 1;  // whole waves: 32 * P threads
       if (wide_valid(g, c)) { *cfg = c; return true; }
@@ -321,11 +323,13 @@ hipError_t launch_mixed32(const Geom &g, const HybridCfg &cfg, int refine, const
                           hipStream_t st) {
   const bool contig = g.stride == 1;
   if (cfg.V == 3) {
-    switch (cfg.M) {
-      case 16: return launch_wide32<16>(g, cfg, refine, a, b, c, d, x, st);
-      case 32: return launch_wide32<32>(g, cfg, refine, a, b, c, d, x, st);
-      default: return hipErrorInvalidValue;
-    }
+    if (cfg.W == 32 && cfg.M == 16)
+      return launch_wide32<16, 32>(g, cfg, refine, a, b, c, d, x, st);
+    if (cfg.W == 32 && cfg.M == 32)
+      return launch_wide32<32, 32>(g, cfg, refine, a, b, c, d, x, st);
+    if (cfg.W == 16 && cfg.M == 32)
+      return launch_wide32<32, 16>(g, cfg, refine, a, b, c, d, x, st);
+    return hipErrorInvalidValue;
   }
   switch (cfg.M) {
     case 8: return contig ? launch_reg<8, true>(g, cfg, refine, a, b, c, d, x, st)
