@@ -273,8 +273,9 @@ bool mixed32_cfg_valid(const Geom &g, const HybridCfg &c) {
 
 // Default shapes, from a sweep at 512^3 (one refinement): contiguous
 // systems 16 rows per lane from 256 rows on (else 8), in workgroups of up to
-// 256 threads (x: 0.69 ms, against 0.74 at 8 rows per lane and 0.85 at 512
-// threads); strided systems the wide kernel at 32 rows per lane with the
+// 256 threads (x: 0.67-0.69 ms, against 0.72-0.74 at 8 rows per lane, 0.78
+// at 4 and 0.85 at 512 threads; 16-byte vector loads measured 1.10-1.27 ms
+// and are not offered); strided systems the wide kernel at 32 rows per lane with the
 // workgroups dealt to the XCDs in 32 groups (y 0.68, z 0.81 ms; 16 rows per
 // lane in 1024 threads spill a few registers and run 0.82 / 1.15 ms), else
 // 16 x-neighbours in registers.
