@@ -1,4 +1,5 @@
-"""Seeded random sweep of the C API's input space against the oracle.
+"""Seeded random sweep of the C API's input space against the oracle
+(plain and Inc solves, both mixed-precision solves, ragged batches).
 
 Each case draws ndim 1-3, extents 1..70 (one dimension sometimes up to 600),
 pads of up to 3 extra elements per dimension, a base offset into the
@@ -106,6 +107,35 @@ def test_random_shape_mixed(case):
     mask = np.ones(total, dtype=bool)
     mask[idx] = False
     assert np.array_equal(got[mask], d[mask])
+
+
+@pytest.mark.parametrize("case", range(30))
+def test_random_shape_mixed_f32(case):
+    """FP64 solution of FP32-stored systems on random shapes, pads and
+    offsets: <= 1e-12 of the long-double oracle's solution of the FP32 values
+    promoted; x keeps what it held outside the systems, and d is not
+    written."""
+    rng = np.random.default_rng(4000 + case)
+    ndim, dims, pads, sd, _, _, _, offset = draw(rng)
+    total = offset + int(np.prod(pads)) + 3
+    a, b, c, d = H.diag_dominant(rng, total, np.float32)
+    t = [torch.from_numpy(v.copy()).cuda() for v in (a, b, c, d)]
+    x0 = torch.full((total,), -5.0, dtype=torch.float64, device="cuda")
+    x = api.mixed_f32_strided_batch(*t, x0, solvedim=sd, dims=dims, pads=pads, offset=offset,
+                                    refinements=1, sync=True).cpu().numpy()
+    sh = lambda v: np.ascontiguousarray(v[offset:offset + int(np.prod(pads))].astype(np.float64))
+    ref = oracle.batch_ld(sh(a), sh(b), sh(c), sh(d), ndim=ndim, solvedim=sd, dims=dims,
+                          pads=pads)
+    want = np.full(total, -5.0)
+    want[offset:offset + int(np.prod(pads))] = ref
+    idx = np.concatenate(list(H.system_slices(ndim, sd, dims, pads, offset)))
+    what = "case %d: ndim %d dims %s pads %s sd %d offset %d config %s" % (
+        case, ndim, dims, pads, sd, offset, api.last_config())
+    H.assert_norm_close(x[idx], want[idx], np.float64, tol=1e-12)
+    mask = np.ones(total, dtype=bool)
+    mask[idx] = False
+    assert np.all(x[mask] == -5.0), what + ": wrote outside the systems"
+    assert np.array_equal(t[3].cpu().numpy(), d), what + ": wrote d"
 
 
 @pytest.mark.parametrize("case", range(30))
