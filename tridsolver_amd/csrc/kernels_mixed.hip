@@ -15,9 +15,11 @@
 //   4. `refine` times: r = d - A x in FP64 (neighbour chunks' boundary x via
 //      LDS), dx = FP32 solve of (float) r, x += dx in FP64;
 //   5. x is written in FP64.
-// The residual and the accumulation are the only FP64 arithmetic; HBM traffic
-// is one read of a, b, c, d and one write of d (40 B per point), the same as
-// the FP64 solve.
+// The residual and the accumulation are the only FP64 arithmetic; the FP32
+// factorisation takes its reciprocals from the hardware's approximate
+// reciprocal (mixed_core.hpp, FAST), whose error the refinement removes. HBM
+// traffic is one read of a, b, c, d and one write of d (40 B per point), the
+// same as the FP64 solve.
 
 #include <algorithm>
 #include <atomic>
@@ -188,7 +190,7 @@ __global__ __launch_bounds__(MIXED_THREADS) void mixed_wide_kernel(
   __syncthreads();
 
   double X[M];
-  mixed_chunk_acc<M, MIXED_MAX_STEPS>(
+  mixed_chunk_acc<M, MIXED_MAX_STEPS, true, false, true>(
       L, ring, [&](int r) { return sa[r * W]; }, B, [&](int r) { return sc[r * W]; }, D, refine,
       X);
   if (!L.active) return;

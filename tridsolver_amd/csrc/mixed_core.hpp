@@ -228,7 +228,7 @@ __device__ __forceinline__ void mixed_chunk(const hyb::Lane &L, hyb::Ring &ring,
                                             const double (&A)[M], const double (&B)[M],
                                             const double (&Cc)[M], const double (&D)[M],
                                             int refine, double (&X)[M]) {
-  mixed_chunk_acc<M, MIXED_MAX_STEPS>(
+  mixed_chunk_acc<M, MIXED_MAX_STEPS, true, false, true>(
       L, ring, [&](int r) { return A[r]; }, B, [&](int r) { return Cc[r]; }, D, refine, X);
 }
 
@@ -278,7 +278,7 @@ __device__ __forceinline__ void mixed_contig_lds(const hyb::Lane &L, hyb::Ring &
   __syncthreads();
   const double *ca = tA + L.p * (M + 1), *cc = tC + L.p * (M + 1);
   double X[M];
-  mixed_chunk_acc<M, STEPS>(
+  mixed_chunk_acc<M, STEPS, true, false, true>(
       L, ring, [&](int r) { return ca[r]; }, B, [&](int r) { return cc[r]; }, D, refine, X);
   hyb::to_fetch<M, true>(L, X, tA);
   // the store base is recomputed behind an empty asm, so that the load
