@@ -4,14 +4,16 @@
 // For each system (same chunking, LDS staging and reduced PCR as the hybrid
 // in kernels_solve.hip):
 //   1. the FP64 operands a, b, c, d of the lane's chunk are loaded once and
-//      kept in registers;
-//   2. the matrix is factored in FP32: the spike sweeps' multipliers
-//      (1/beta_i, ga_i, ap_i, gp_i per row) and, for every PCR step, the
-//      pre-step coefficients (A, C, 1/(1 - A Cm - C Ap)) -- none of these
-//      depend on the right-hand side;
+//      kept on chip (registers; a and c in LDS for the M = 16 kernels);
+//   2. the matrix is factored in FP32 (mixed_core.hpp): the forward sweep's
+//      multipliers (1/beta_i, ga_i and the spike al_i per row), the chunk's
+//      first row in reduced form and, for every PCR step, the pre-step
+//      coefficients (A, C, 1/(1 - A Cm - C Ap)) -- none of these depend on
+//      the right-hand side;
 //   3. x0 = FP32 solve of (float) d with that factorisation (right-hand-side
-//      recurrences only: de_i = (rhs_i - a_i de_{i-1}) / beta_i, backward
-//      dp_i = de_i - ga_i dp_{i+1}, PCR on D only, substitution);
+//      recurrences only: de_i = (rhs_i - a_i de_{i-1}) / beta_i, the first
+//      row's dp_0, PCR on D only, then x_i = de_i - al_i x_{s-1} -
+//      ga_i x_{i+1} from the chunk's end);
 //   4. `refine` times: r = d - A x in FP64 (neighbour chunks' boundary x via
 //      LDS), dx = FP32 solve of (float) r, x += dx in FP64;
 //   5. x is written in FP64.
