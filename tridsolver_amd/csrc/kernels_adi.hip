@@ -300,7 +300,10 @@ bool adi_cfg(int dev, const Geom &g, HybridCfg *cfg) {
 // 0.217 against 0.229-0.233 ms at M = 4; FP32 0.149 against 0.159-0.162).
 // G = 1 and 8 time alike; with G = 8 neighbouring x lines run on one XCD,
 // so the halo lines the right-hand side reads (j +- 1) hit in its L2
-// instead of being fetched again.
+// instead of being fetched again. Re-measured with the branch-free
+// right-hand-side loads: M = 16 / T = 64, M = 8 / T = 128, M = 16 / T = 128
+// and M = 4 / T = 64 take the FP64 step to 0.206-0.215 ms against 0.200, and
+// the FP32 step to 0.134-0.160 ms against 0.133.
 template <typename REAL>
 struct AdiX {
   static constexpr int M = 8;
