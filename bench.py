@@ -18,7 +18,7 @@ point, FP64 solution); adjoint / forward time on 256^3 FP64; the ragged
 40,000-system batch; the hipGraph-replayed ADI step on 256^3; one 8-way
 z-slab shard of the partitioned solve.
 
-Prints ONE JSON line (the driver's contract).
+Prints ONE JSON line.
 """
 from __future__ import annotations
 

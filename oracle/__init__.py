@@ -1,8 +1,8 @@
 """CPU oracle for the tridsolver_amd parity tests.
 
 TEST INFRASTRUCTURE ONLY: tests/, ``__graft_entry__.smoke()`` and bench.py's
-cpu_baseline leg use this package as the checker. The product library never
-imports it.
+cpu_baseline leg use this package to check and time against. The product
+library never imports it.
 
 Two libraries are wrapped:
 

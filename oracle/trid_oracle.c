@@ -1,8 +1,8 @@
 /*
  * trid_oracle.c - CPU oracle for the tridsolver_amd parity tests.
  *
- * TEST INFRASTRUCTURE ONLY. This file is the checker: only tests/,
- * __graft_entry__.smoke() and bench.py's cpu_baseline leg load it. The
+ * TEST INFRASTRUCTURE ONLY: only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg load it, to check and time against. The
  * product library (tridsolver_amd/lib/libtridhip.so) never links or calls it.
  *
  * It restates, in plain C, the reference's CPU algorithm for the scalar batched

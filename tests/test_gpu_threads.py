@@ -2,7 +2,7 @@
 
 The library keeps the stream, the forced shapes and the last error per
 calling thread, and guards its shared state (device checks, workspaces,
-tuning cache, dynamic-LDS grants) with locks or atomics. Four threads solve
+tuning cache, dynamic-LDS grants) with locks or atomics. Five threads solve
 different batches concurrently -- hybrid solves along x, y and z, the
 per-lane path (which uses a per-stream workspace), ragged batches -- and one
 of them forces a hybrid shape; every result must match the oracle, and the
@@ -55,7 +55,7 @@ def test_threads_on_own_streams():
         ("ragged", _ragged_case(4, np.float64), {}),
         ("forced", _solve_case(5, [256, 30, 20], 0, np.float64), dict(sd=0, dims=[256, 30, 20])),
     ]
-    results, configs, errors = {}, {}, []
+    results, shapes, errors = {}, {}, []
     start = threading.Barrier(len(jobs))
 
     def work(k, kind, case, kw):
@@ -81,7 +81,7 @@ def test_threads_on_own_streams():
                     else:
                         api.trid_strided_batch(*dev[:3], td, solvedim=kw["sd"], dims=kw["dims"],
                                                stream=stream)
-                    configs.setdefault(k, []).append(api.last_config())
+                    shapes.setdefault(k, []).append(api.last_config())
                 stream.synchronize()
             results[k] = td.cpu().numpy()
         except Exception as e:  # reported by the main thread
@@ -98,12 +98,12 @@ def test_threads_on_own_streams():
         arrays, want = case
         dtype = arrays[3].dtype
         H.assert_norm_close(results[k], want, dtype)
-        algos = {cfg[0] for cfg in configs[k]}
+        algos = {cfg[0] for cfg in shapes[k]}
         if kind == "lane":
-            assert algos == {1}, configs[k]
+            assert algos == {1}, shapes[k]
         elif kind == "ragged":
-            assert algos == {5}, configs[k]
+            assert algos == {5}, shapes[k]
         elif kind == "forced":
-            assert {cfg[:4] for cfg in configs[k]} == {(3, 4, 64, 4)}, configs[k]
+            assert {cfg[:4] for cfg in shapes[k]} == {(3, 4, 64, 4)}, shapes[k]
         else:  # the other thread's forced shape is not seen here
-            assert all(cfg[1:4] != (4, 64, 4) for cfg in configs[k]), configs[k]
+            assert all(cfg[1:4] != (4, 64, 4) for cfg in shapes[k]), shapes[k]
