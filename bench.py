@@ -472,8 +472,8 @@ def bench_adi(torch, n=256, iters=20):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only (the solve has no "
-              "multi-GPU path); run with --gpus 1", file=sys.stderr)
+        print("bench.py: measures one GPU only (the library has no multi-GPU path); "
+              "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     import torch
 
