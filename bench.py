@@ -427,8 +427,7 @@ def bench_partitioned(torch, api, n=512, shards=8):
     del a, b, c, d, x, bnd, bnd_all
     torch.cuda.empty_cache()
     return {"shard": dims, "shards": shards, "boundary_ms": round(t_bnd, 4),
-            # algorithmic bytes: a, b, c, d once (the bottom-up sweep's
-            # re-read of the same rows is served by the caches)
+            # algorithmic bytes: a, b, c, d, read once by the one-pass sweep
             "boundary_alg_GBs": round(32.0 * npts / t_bnd / 1e6, 1),
             "finish_ms": round(t_fin, 4), "local_solve_ms": round(t_loc, 4),
             "partitioned_over_local": round((t_bnd + t_fin) / t_loc, 3),

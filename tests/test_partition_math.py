@@ -56,6 +56,20 @@ def boundary(a, b, c, d, first, last):
             -a[n - 1] * aam / bE, cn / bE, (d[n - 1] - a[n - 1] * ddm) / bE)
 
 
+def row1_forward(a, b, c, d):
+    """Row 1 of a shard as x_1 = dd1 - aa1 F - cc1 E from the top-down sweep
+    alone: each new row x_i = D_i - A_i F - C_i x_{i+1} is substituted into
+    x_1 = P + Q F + R x_i (the GPU boundary step's one-pass form)."""
+    n = len(b)
+    D, A, C = d[1] / b[1], a[1] / b[1], c[1] / b[1]
+    P, Q, R = D, -A, -C
+    for i in range(2, n - 1):
+        beta = b[i] - a[i] * C
+        D, A, C = (d[i] - a[i] * D) / beta, -a[i] * A / beta, c[i] / beta
+        P, Q, R = P + R * D, Q - R * A, -R * C
+    return P, -Q, -R
+
+
 def reduced_rows(recs):
     """(sub, sup, rhs) of the 2G-row system, unit diagonal."""
     sub, sup, rhs = [], [], []
@@ -134,6 +148,31 @@ def test_boundary_of_two_and_three_rows():
     bE = 6 - 6 / 5
     assert r3 == pytest.approx((0.0, -(2 / 5) / b0, (7 - 8 / 5) / b0,
                                 -(6 / 5) / bE, 0.0, (9 - 24 / 5) / bE))
+
+
+@pytest.mark.parametrize("n", [3, 4, 5, 17, 64, 200])
+def test_row1_from_the_top_down_sweep_alone(n):
+    """The one-pass form of row 1 (the GPU boundary step's) equals the
+    bottom-up sweep's and the dense truth: with the interior matrix A_int of
+    rows 1..n-2, x_int = A_int^-1 (d_int - a_1 F e_first - c_{n-2} E e_last),
+    so dd1 = (A_int^-1 d_int)_first, aa1 = a_1 (A_int^-1 e_first)_first,
+    cc1 = c_{n-2} (A_int^-1 e_last)_first."""
+    rng = np.random.default_rng(n)
+    a, b, c, d = system(rng, n)
+    fwd = row1_forward(a, b, c, d)
+    ai, bi, ci = a[1:n - 1], b[1:n - 1], c[1:n - 1]
+    A = np.diag(bi) + np.diag(ai[1:], -1) + np.diag(ci[:-1], 1)
+    m = n - 2
+    e_first, e_last = np.eye(m)[0], np.eye(m)[m - 1]
+    truth = (np.linalg.solve(A, d[1:n - 1])[0], a[1] * np.linalg.solve(A, e_first)[0],
+             c[n - 2] * np.linalg.solve(A, e_last)[0])
+    assert fwd == pytest.approx(truth, rel=1e-12, abs=1e-14)
+    # the bottom-up sweep of the two-sweep form gives the same numbers
+    D, C, Aa = d[n - 2] / b[n - 2], c[n - 2] / b[n - 2], a[n - 2] / b[n - 2]
+    for i in range(n - 3, 0, -1):
+        gam = b[i] - c[i] * Aa
+        D, C, Aa = (d[i] - c[i] * D) / gam, -c[i] * C / gam, a[i] / gam
+    assert fwd == pytest.approx((D, Aa, C), rel=1e-12, abs=1e-14)
 
 
 def test_twisted_matches_dense_reduced():

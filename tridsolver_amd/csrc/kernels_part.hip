@@ -6,8 +6,11 @@
 //
 // Per shard k of n >= 2 rows, F = its first unknown, E = its last:
 //  boundary  interior rows 1..n-2 are written as x_i = dd_i - aa_i F - cc_i E
-//            by a top-down sweep (gives row n-2) and a bottom-up sweep (gives
-//            row 1), neither storing anything; rows 0 and n-1 then read,
+//            by one top-down sweep that stores nothing: its recurrence gives
+//            row n-2 (x_i = D_i - A_i F - C_i x_{i+1}), and substituting each
+//            new row into row 1's running form x_1 = P + Q F + R x_{i+1}
+//            (P += R D_i, Q -= R A_i, R = -R C_i) gives row 1 at the end, with
+//            x_{n-1} = E; rows 0 and n-1 then read,
 //            normalised by their diagonal,
 //                alpha E_{k-1} + F + gamma E       = delta
 //                alpha' F      + E + gamma' F_{k+1} = delta'
@@ -23,7 +26,7 @@
 //            are solved by the batched solve (any shape it picks). For Inc,
 //            the two rows of d are saved before and restored after, so d
 //            stays untouched as in the plain Inc solve.
-// Traffic per shard point: boundary reads a, b, c, d twice (64 B FP64), the
+// Traffic per shard point: boundary reads a, b, c, d once (32 B FP64), the
 // solve moves its usual 40 (48 Inc); the exchanged data is 6 reals per
 // system and shard.
 
@@ -55,8 +58,10 @@ __global__ __launch_bounds__(PART_BLOCK) void part_boundary_kernel(
     dd1 = 0; aa1 = 0; cc1 = -1;  // row 1 is E itself
     ddm = 0; aam = -1; ccm = 0;  // row n-2 is F itself
   } else {
-    // top-down over rows 1..n-2: x_i = D - A F - C x_{i+1}
+    // top-down over rows 1..n-2: x_i = D - A F - C x_{i+1}; row 1 carried
+    // along as x_1 = P + Q F + R x_{i+1}
     REAL D = d[str] / b[str], A = a[str] / b[str], C = c[str] / b[str];
+    REAL P = D, Q = -A, R = -C;
     int64_t i = 2;
     for (; i + PART_K <= n - 1; i += PART_K) {
       REAL av[PART_K], bv[PART_K], cv[PART_K], dv[PART_K];
@@ -71,6 +76,9 @@ __global__ __launch_bounds__(PART_BLOCK) void part_boundary_kernel(
         D = (dv[k] - av[k] * D) * inv;
         A = -av[k] * A * inv;
         C = cv[k] * inv;
+        P += R * D;
+        Q -= R * A;
+        R = -R * C;
       }
     }
     for (; i < n - 1; ++i) {
@@ -80,36 +88,12 @@ __global__ __launch_bounds__(PART_BLOCK) void part_boundary_kernel(
       D = (d[o] - ai * D) * inv;
       A = -ai * A * inv;
       C = c[o] * inv;
+      P += R * D;
+      Q -= R * A;
+      R = -R * C;
     }
     ddm = D; aam = A; ccm = C;
-    // bottom-up over rows n-2..1: x_i = D' - C' E - A' x_{i-1}
-    const int64_t om = (n - 2) * str;
-    D = d[om] / b[om]; C = c[om] / b[om]; A = a[om] / b[om];
-    i = n - 3;
-    for (; i - PART_K + 1 >= 1; i -= PART_K) {
-      REAL av[PART_K], bv[PART_K], cv[PART_K], dv[PART_K];
-#pragma unroll
-      for (int k = 0; k < PART_K; ++k) {
-        const int64_t o = (i - k) * str;
-        av[k] = a[o]; bv[k] = b[o]; cv[k] = c[o]; dv[k] = d[o];
-      }
-#pragma unroll
-      for (int k = 0; k < PART_K; ++k) {
-        const REAL inv = REAL(1) / (bv[k] - cv[k] * A);
-        D = (dv[k] - cv[k] * D) * inv;
-        C = -cv[k] * C * inv;
-        A = av[k] * inv;
-      }
-    }
-    for (; i >= 1; --i) {
-      const int64_t o = i * str;
-      const REAL ci = c[o];
-      const REAL inv = REAL(1) / (b[o] - ci * A);
-      D = (d[o] - ci * D) * inv;
-      C = -ci * C * inv;
-      A = a[o] * inv;
-    }
-    dd1 = D; aa1 = A; cc1 = C;
+    dd1 = P; aa1 = -Q; cc1 = -R;  // x_1 = dd1 - aa1 F - cc1 E
   }
   const int64_t oe = (n - 1) * str;
   const REAL a0 = first ? REAL(0) : a[0];
