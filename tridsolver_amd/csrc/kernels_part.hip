@@ -20,10 +20,11 @@
 //            (an allgather over RCCL in a distributed run);
 //  solve     the 2G unknowns (F_0, E_0, F_1, ...) form a tridiagonal system
 //            with unit diagonal; each system's lane gets E_{k-1} and F_{k+1}
-//            by a twisted elimination (top-down and bottom-up sweeps meeting
-//            at the wanted row; O(G), no storage), subtracts a_0 E_{k-1} from
-//            d_0 and c_{n-1} F_{k+1} from d_{n-1}, and the shard's own rows
-//            are solved by the batched solve (any shape it picks). For Inc,
+//            by twisted elimination (top-down and bottom-up sweeps meeting
+//            at the wanted rows, one pass each for both; O(G), no storage),
+//            subtracts a_0 E_{k-1} from d_0 and c_{n-1} F_{k+1} from
+//            d_{n-1}, and the shard's own rows are solved by the batched
+//            solve (any shape it picks). For Inc,
 //            the two rows of d are saved before and restored after, so d
 //            stays untouched as in the plain Inc solve.
 // Traffic per shard point: boundary reads a, b, c, d once (32 B FP64), the
@@ -145,6 +146,39 @@ __device__ REAL twisted(const REAL *__restrict__ all, int64_t ns, int64_t s, int
   return (rhs - sub * df - sup * dr) / (REAL(1) - sub * cf - sup * ar);
 }
 
+// Unknowns j1 < j2 of the reduced system at once: one top-down sweep over
+// rows 0..j2-1 (its state before row j1 kept) and one bottom-up sweep over
+// rows R-1..j1+1 (its state before row j2 kept), so each row is read once
+// (rows j1..j2 twice). Each unknown gets exactly the operations of
+// twisted(j), hence the same value.
+template <typename REAL>
+__device__ void twisted2(const REAL *__restrict__ all, int64_t ns, int64_t s, int R, int j1,
+                         int j2, REAL &x1, REAL &x2) {
+  REAL cf = 0, df = 0, cf1 = 0, df1 = 0;
+  for (int i = 0; i < j2; ++i) {
+    if (i == j1) { cf1 = cf; df1 = df; }
+    REAL sub, sup, rhs;
+    reduced_row(all, ns, s, i, sub, sup, rhs);
+    const REAL inv = REAL(1) / (REAL(1) - sub * cf);
+    df = (rhs - sub * df) * inv;
+    cf = sup * inv;
+  }
+  REAL ar = 0, dr = 0, ar2 = 0, dr2 = 0;
+  for (int i = R - 1; i > j1; --i) {
+    if (i == j2) { ar2 = ar; dr2 = dr; }
+    REAL sub, sup, rhs;
+    reduced_row(all, ns, s, i, sub, sup, rhs);
+    const REAL inv = REAL(1) / (REAL(1) - sup * ar);
+    dr = (rhs - sup * dr) * inv;
+    ar = sub * inv;
+  }
+  REAL sub, sup, rhs;
+  reduced_row(all, ns, s, j1, sub, sup, rhs);
+  x1 = (rhs - sub * df1 - sup * dr) / (REAL(1) - sub * cf1 - sup * ar);
+  reduced_row(all, ns, s, j2, sub, sup, rhs);
+  x2 = (rhs - sub * df - sup * dr2) / (REAL(1) - sub * cf - sup * ar2);
+}
+
 template <typename REAL, bool INC>
 __global__ __launch_bounds__(PART_BLOCK) void part_adjust_kernel(
     Geom g, const REAL *__restrict__ a, const REAL *__restrict__ c, REAL *__restrict__ d,
@@ -160,6 +194,13 @@ __global__ __launch_bounds__(PART_BLOCK) void part_adjust_kernel(
     save[ns + s] = dE;
   }
   const int R = 2 * G;
+  if (k > 0 && k < G - 1) {  // an interior shard needs E_{k-1} and F_{k+1}
+    REAL e_prev, f_next;
+    twisted2(all, ns, s, R, 2 * k - 1, 2 * k + 2, e_prev, f_next);
+    d[base] = d0 - a[base] * e_prev;
+    d[oe] = dE - c[oe] * f_next;
+    return;
+  }
   if (k > 0) d[base] = d0 - a[base] * twisted(all, ns, s, R, 2 * k - 1);
   if (k < G - 1) d[oe] = dE - c[oe] * twisted(all, ns, s, R, 2 * k + 2);
 }
