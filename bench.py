@@ -399,6 +399,20 @@ def bench_ragged(torch, api, tune):
     out["mixed"] = dict(runs[nref], refinements=nref, config=api.last_config(),
                         by_refinements={str(k): v for k, v in runs.items()})
     del a, b, c, d, ref, x
+    # the same batch stored in FP32, FP64 solution (one refinement): 24 B per
+    # point; error against the FP64 solve of the FP32 values promoted
+    a, b, c, d = gen_gpu(torch, total, torch.float32, seed=8192)
+    ref = d.double()
+    api.ragged_batch(a.double(), b.double(), c.double(), ref, toff, max_len=8192, sync=True)
+    x = api.ragged_batch_mixed_f32(a, b, c, d, toff, max_len=8192, refinements=1, sync=True)
+    err = float((x - ref).abs().max() / ref.abs().max())
+    ms = event_ms(torch, lambda: api.ragged_batch_mixed_f32(a, b, c, d, toff, x, max_len=8192,
+                                                            refinements=1))
+    out["mixed_f32_stored"] = {"ms": round(ms, 4), "speedup_vs_fp64": round(ms64 / ms, 3),
+                               "alg_GBs": round((24.0 * total + 8 * len(off)) / ms / 1e6, 1),
+                               "max_rel_err_vs_fp64": err, "refinements": 1,
+                               "config": api.last_config()}
+    del a, b, c, d, ref, x
     torch.cuda.empty_cache()
     return out
 

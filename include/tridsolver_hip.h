@@ -46,7 +46,8 @@ EXTERN_C size_t tridHipWorkspaceBytes(int fp64, int inc, int ndim, int solvedim,
 /* What the last solve of this thread ran: algorithm (1 per-lane Thomas,
  * 2 mixed precision, 3 hybrid, 4 adjoint, 5 ragged, 6 streamed strided
  * hybrid, 7 streamed strided hybrid with LDS storage, 8 wide strided hybrid,
- * 9 ragged mixed precision, 10 FP64 solution of FP32-stored systems), and
+ * 9 ragged mixed precision, 10 FP64 solution of FP32-stored systems, 11 the
+ * same for a ragged batch), and
  * for the hybrid its
  * rows per lane M, lanes per system P, systems per workgroup W and XCD
  * grouping G (consecutive system groups dealt to one XCD; 1 = plain order).
@@ -189,6 +190,18 @@ EXTERN_C tridStatus_t tridDmtsvRaggedBatchMixed(const TridParams *ctx, const dou
                                                 const double *b, const double *c, double *d,
                                                 const int64_t *offsets, int64_t nsys,
                                                 int64_t max_len, int refinements);
+/* FP64 solution of a ragged batch stored in FP32 (as
+ * tridDmtsvStridedBatchMixedF32 for strided batches): FP32 a, b, c, d with
+ * the layout of tridSmtsvRaggedBatch, FP64 x with the same offsets; d is not
+ * written. FP32 factorisation and sweeps, FP64 residual and accumulation,
+ * `refinements` (0..8) times, in one persistent launch after the planning
+ * kernels: 16 B read and 8 B written per point. Lengths up to 8192;
+ * otherwise as tridDmtsvRaggedBatch. */
+EXTERN_C tridStatus_t tridDmtsvRaggedBatchMixedF32(const TridParams *ctx, const float *a,
+                                                  const float *b, const float *c,
+                                                  const float *d, double *x,
+                                                  const int64_t *offsets, int64_t nsys,
+                                                  int64_t max_len, int refinements);
 EXTERN_C size_t tridHipRaggedWorkspaceBytes(int64_t nsys);
 /* Forces the ragged kernel's rows per lane (4, 8 or 16), workgroups per CU
  * and solve launches (1; or 2: systems needing more than half a workgroup's

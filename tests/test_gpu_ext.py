@@ -463,6 +463,35 @@ def test_ragged_vs_oracle(dtype, M, launches):
             ctx.__exit__(None, None, None)
 
 
+@pytest.mark.parametrize("max_len", [2048, 8192])
+def test_ragged_mixed_f32_vs_oracle(max_len):
+    """FP64 solution of a ragged batch stored in FP32 (empty and one-row
+    systems included): 1 and 2 refinements within 1e-12 of the FP64 oracle's
+    solution of the promoted values, 0 refinements FP32-accurate; d is not
+    written and x keeps its value past the last system."""
+    rng = np.random.default_rng(max_len + 9)
+    lens = np.round(np.exp(rng.uniform(np.log(1), np.log(max_len), 1500))).astype(np.int64)
+    lens[:5] = [0, 1, 2, 0, max_len]
+    off, a, b, c, d = ragged_case(rng, lens, np.float32)
+    # three elements past the last system, which no system owns
+    a, b, c, d = (np.concatenate([v, np.full(3, 2.0, np.float32)]) for v in (a, b, c, d))
+    want = d.astype(np.float64)
+    oracle.ragged(*(v.astype(np.float64) for v in (a, b, c)), want, off)
+    t = [dev(v) for v in (a, b, c, d)]
+    toff = dev(off)
+    for refine, tol in ((0, 1e-5), (1, 1e-12), (2, 1e-12)):
+        x0 = torch.full((len(d),), 3.0, dtype=torch.float64, device="cuda")
+        x = api.ragged_batch_mixed_f32(*t, toff, x0, max_len=max_len, refinements=refine,
+                                       sync=True).cpu().numpy()
+        assert api.last_config()[0] == 11
+        n = int(off[-1])
+        err = H.assert_norm_close(x[:n], want[:n], np.float64, tol=tol)
+        if refine == 0:
+            assert err > 1e-12
+        assert np.all(x[n:] == 3.0)
+    assert np.array_equal(t[3].cpu().numpy(), d)
+
+
 def test_ragged_too_long_is_reported():
     rng = np.random.default_rng(4)
     off, a, b, c, d = ragged_case(rng, [10, 9000, 5], np.float64)

@@ -72,6 +72,7 @@ SIGNATURES = {
     "tridSmtsvRaggedBatch": ([P] * 6 + [L, L], I),
     "tridDmtsvRaggedBatchInc": ([P] * 7 + [L, L], I),
     "tridDmtsvRaggedBatchMixed": ([P] * 6 + [L, L, I], I),
+    "tridDmtsvRaggedBatchMixedF32": ([P] * 7 + [L, L, I], I),
     "tridDmtsvStridedBatchPartBoundary": ([P] * 5 + [I, I, P, P, I, I, P], I),
     "tridSmtsvStridedBatchPartBoundary": ([P] * 5 + [I, I, P, P, I, I, P], I),
     "tridDmtsvStridedBatchPartSolve": ([P] * 5 + [I, I, P, P, I, I, P], I),

@@ -277,6 +277,29 @@ def ragged_batch_mixed(a, b, c, d, offsets, *, max_len, refinements=2, sync=Fals
     return d
 
 
+def ragged_batch_mixed_f32(a, b, c, d, offsets, x=None, *, max_len, refinements=1, sync=False,
+                           stream=None):
+    """FP64 solution of a ragged batch stored in FP32 (tridDmtsvRaggedBatchMixedF32):
+    a, b, c, d float32, x float64 of d's size (allocated when None); returns
+    x. Lengths up to 8192."""
+    if d.dtype != torch.float32:
+        raise TypeError("the FP32-stored ragged solve takes float32 operands")
+    if offsets.dtype != torch.int64 or offsets.device != d.device:
+        raise TypeError("offsets must be an int64 tensor on the arrays' device")
+    x = torch.empty(d.shape, dtype=torch.float64, device=d.device) if x is None else x
+    if x.dtype != torch.float64 or x.numel() != d.numel() or x.device != d.device:
+        raise TypeError("x must be a float64 tensor of d's size on d's device")
+    _validate([a, b, c, d], [d.numel()], [d.numel()], 0)
+    _validate([x], [d.numel()], [d.numel()], 0)
+    use_torch_stream(stream)
+    params = make_params((0, 0, 0), 1 if sync else 0)
+    args = [_ptr(t) for t in (a, b, c, d, x)] + [_ptr(offsets)]
+    st = _lib.lib().tridDmtsvRaggedBatchMixedF32(C.byref(params), *args, offsets.numel() - 1,
+                                                 int(max_len), int(refinements))
+    check(st, "tridDmtsvRaggedBatchMixedF32")
+    return x
+
+
 class forced_ragged:
     """Context manager forcing the ragged kernel's rows per lane, workgroups
     per CU and solve launches (1, or 2 with the short systems apart) for this

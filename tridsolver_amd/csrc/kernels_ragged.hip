@@ -348,6 +348,77 @@ __global__ __launch_bounds__(RG_MIXED_THREADS) void rg_solve_mixed(
   }
 }
 
+// FP64 solution of FP32-stored systems (the arithmetic of kernels_mixed32.hip):
+// FP32 a, b, c, d in, FP64 x out (d is not written); 16 rows per lane in
+// 512 threads, so systems up to 8192 rows; ONE: at most one refinement (the
+// general loop spills at this shape). The LDS tile holds doubles for x's
+// transpose; the FP32 operands go through its first half.
+constexpr int RG_M32_M = 16;
+
+template <bool ONE>
+__global__ __launch_bounds__(RG_MIXED_THREADS) void rg_solve_mixed32(
+    const float *__restrict__ a, const float *__restrict__ b, const float *__restrict__ c,
+    const float *__restrict__ d, double *__restrict__ x, const int64_t *__restrict__ off,
+    const int *__restrict__ list, RaggedPlan *plan, int refine) {
+  constexpr int M = RG_M32_M, Tmax = RG_MIXED_THREADS;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ int s_next;
+  const int t = threadIdx.x;
+  const int total = plan->total_items;
+  if (t == 0) s_next = atomicAdd(&plan->work, 1);
+  __syncthreads();
+  int item = s_next;
+  RgMeta cur = item < total ? rg_meta(plan, list, off, item, t, Tmax) : RgMeta{};
+  for (;;) {
+    __syncthreads();  // every thread has read s_next
+    if (item >= total) break;
+    if (t == 0) s_next = atomicAdd(&plan->work, 1);
+    const hyb::Lane L = rg_lane(cur, t, Tmax);
+    double *dtile = reinterpret_cast<double *>(smem_raw) + (size_t)L.w * L.P * (M + 1);
+    float *ftile = reinterpret_cast<float *>(dtile);
+    hyb::Ring ring{smem_raw + sizeof(double) * (size_t)Tmax * (M + 1),
+                   (int)(2 * sizeof(double) * Tmax), 0};
+    float A[M], B[M], Cc[M], D[M];
+    hyb::fetch_abcd<M, true>(L, a, b, c, d, A, B, Cc, D);
+    hyb::to_chunk<M, true>(L, A, ftile);  // has barriers: s_next is visible below
+    const int nxt = s_next;
+    const RgMeta next = nxt < total ? rg_meta(plan, list, off, nxt, t, Tmax) : RgMeta{};
+    hyb::to_chunk<M, true>(L, B, ftile);
+    hyb::to_chunk<M, true>(L, Cc, ftile);
+    hyb::to_chunk<M, true>(L, D, ftile);
+    double X[M];
+    mixed_chunk_acc<M, MIXED_MAX_STEPS, false, ONE, true>(
+        L, ring, [&](int r) { return A[r]; }, B, [&](int r) { return Cc[r]; }, D, refine, X);
+    hyb::to_fetch<M, true>(L, X, dtile);
+    hyb::store<M, true>(L, x, X);
+    item = nxt;
+    cur = next;
+  }
+}
+
+template <bool ONE>
+hipError_t launch_mixed32_one(const float *a, const float *b, const float *c, const float *d,
+                              double *x, const int64_t *off, int64_t S, int per_cu, int refine,
+                              void *ws, hipStream_t st, int dev) {
+  constexpr int Tmax = RG_MIXED_THREADS;
+  RaggedPlan *plan = static_cast<RaggedPlan *>(ws);
+  int *list = reinterpret_cast<int *>(static_cast<unsigned char *>(ws) + 512);
+  launch_plan(off, S, RG_M32_M, Tmax, Tmax, ws, st);  // one launch: no long range
+  const size_t lds =
+      sizeof(double) * (size_t)Tmax * (RG_M32_M + 1) + hyb::ring_bytes<double>(Tmax, 2);
+  auto kern = rg_solve_mixed32<ONE>;
+  static LdsGrant lds_granted;
+  {
+    hipError_t e = ensure_lds(reinterpret_cast<const void *>(kern), lds, lds_granted);
+    if (e != hipSuccess) return e;
+  }
+  int cus = 256;
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  hipLaunchKernelGGL(kern, dim3((unsigned)(cus * std::max(1, per_cu))), dim3(Tmax), lds, st,
+                     a, b, c, d, x, off, list, plan, refine);
+  return hipGetLastError();
+}
+
 template <int M>
 hipError_t launch_mixed_m(const double *a, const double *b, const double *c, double *d,
                           const int64_t *off, int64_t S, int per_cu, int refine, void *ws,
@@ -462,6 +533,16 @@ hipError_t launch_mixed_lds(const double *a, const double *b, const double *c, d
 }
 
 }  // namespace
+
+int ragged_mixed32_max_len() { return RG_M32_M * RG_MIXED_THREADS; }
+
+hipError_t launch_ragged_mixed32(const float *a, const float *b, const float *c, const float *d,
+                                 double *x, const int64_t *off, int64_t S, int per_cu,
+                                 int refine, void *ws, hipStream_t st, int dev) {
+  return refine <= 1
+             ? launch_mixed32_one<true>(a, b, c, d, x, off, S, per_cu, refine, ws, st, dev)
+             : launch_mixed32_one<false>(a, b, c, d, x, off, S, per_cu, refine, ws, st, dev);
+}
 
 int ragged_mixed_max_len(int M) {
   if (M == RG_MIXED_LDS_M) return M * 512;

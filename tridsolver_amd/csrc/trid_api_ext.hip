@@ -106,6 +106,34 @@ tridStatus_t adjoint(const TridParams *ctx, const REAL *a, const REAL *b, const 
   return finish(ctx, st, launch_adjoint<REAL>(g, cfg, a, b, c, x, w, lam, ga, gb, gc, st));
 }
 
+tridStatus_t ragged_mixed32(const TridParams *ctx, const float *a, const float *b,
+                            const float *c, const float *d, double *x, const int64_t *offsets,
+                            int64_t nsys, int64_t max_len, int refine) {
+  if (!a || !b || !c || !d || !x || !offsets || nsys < 0 || nsys > INT32_MAX || max_len < 0 ||
+      refine < 0 || refine > 8)
+    return TRID_STATUS_INVALID_VALUE;
+  int dev = 0;
+  tridStatus_t s = check_device(&dev);
+  if (s != TRID_STATUS_SUCCESS) return s;
+  hipStream_t st = current_stream();
+  if (nsys == 0) return finish(ctx, st, hipSuccess);
+  if (max_len > ragged_mixed32_max_len()) return TRID_STATUS_INVALID_VALUE;
+  void *ws = nullptr;
+  if ((s = get_workspace(dev, st, ragged_workspace_bytes(nsys), &ws)) != TRID_STATUS_SUCCESS)
+    return s;
+  HybridCfg rec;
+  rec.M = 16; rec.P = 0; rec.W = 1; rec.G = tl_ragged_per_cu;
+  record_config(11, &rec);
+  s = finish(ctx, st, launch_ragged_mixed32(a, b, c, d, x, offsets, nsys, tl_ragged_per_cu,
+                                            refine, ws, st, dev));
+  if (s == TRID_STATUS_SUCCESS && (ctx == nullptr || ctx->sync != 0)) {
+    int flag = 0;
+    if (ragged_error_flag(ws, &flag) != hipSuccess) return TRID_STATUS_EXECUTION_FAILED;
+    if (flag) return TRID_STATUS_INVALID_VALUE;  // a system longer than supported
+  }
+  return s;
+}
+
 }  // namespace
 }  // namespace trid
 
@@ -229,6 +257,14 @@ EXTERN_C tridStatus_t tridHipForceRaggedConfig(int M, int per_cu, int launches) 
 }
 
 EXTERN_C size_t tridHipRaggedWorkspaceBytes(int64_t nsys) { return ragged_workspace_bytes(nsys); }
+
+EXTERN_C tridStatus_t tridDmtsvRaggedBatchMixedF32(const TridParams *ctx, const float *a,
+                                                  const float *b, const float *c,
+                                                  const float *d, double *x,
+                                                  const int64_t *offsets, int64_t nsys,
+                                                  int64_t max_len, int refinements) {
+  return ragged_mixed32(ctx, a, b, c, d, x, offsets, nsys, max_len, refinements);
+}
 
 EXTERN_C tridStatus_t tridDmtsvRaggedBatchMixed(const TridParams *ctx, const double *a,
                                                 const double *b, const double *c, double *d,
