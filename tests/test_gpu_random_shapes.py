@@ -142,7 +142,8 @@ def test_random_shape_mixed_f32(case):
 def test_random_ragged(case):
     """Random ragged batches: 1..300 systems of 0..8192 rows (log-uniform, with
     empty ones), FP32/FP64, plain/Inc, default or forced rows per lane in one
-    or two solve launches, and the mixed-precision ragged solve for FP64."""
+    or two solve launches, and the mixed-precision ragged solves (FP64-stored
+    and FP32-stored)."""
     rng = np.random.default_rng(3000 + case)
     S = int(rng.integers(1, 301))
     lens = np.round(np.exp(rng.uniform(0, np.log(8192), S))).astype(np.int64)
@@ -184,3 +185,12 @@ def test_random_ragged(case):
         api.ragged_batch_mixed(dev(a), dev(b), dev(c), tm, dev(off), max_len=max(max_len, 1),
                                refinements=2, sync=True)
         H.assert_norm_close(tm.cpu().numpy(), want, dtype, tol=1e-12)
+    if dtype == np.float32 and not inc and max_len <= 8192:
+        # FP64 solution of the FP32-stored batch against the oracle's FP64 solve
+        # of the promoted values
+        w64 = d.astype(np.float64)
+        oracle.ragged(*(v.astype(np.float64) for v in (a, b, c)), w64, off)
+        x = api.ragged_batch_mixed_f32(dev(a), dev(b), dev(c), dev(d), dev(off),
+                                       max_len=max(max_len, 1), refinements=1, sync=True)
+        n = int(off[-1])
+        H.assert_norm_close(x.cpu().numpy()[:n], w64[:n], np.float64, tol=1e-12)
