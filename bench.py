@@ -299,7 +299,7 @@ def bench_mixed(torch, api, n, fp64_ms, tune):
     return out
 
 
-def bench_mixed_f32(torch, api, n, fp64_ms):
+def bench_mixed_f32(torch, api, n, fp64_ms, tune=False):
     """FP64 solution of the n^3 system stored in FP32 (tridDmtsvStridedBatchMixedF32):
     16 B read + 8 B written per point. Speed is compared with the FP64 solve
     of the same size (fp64_ms, 40 B per point); accuracy with the FP64 solve
@@ -311,6 +311,8 @@ def bench_mixed_f32(torch, api, n, fp64_ms):
     x = torch.empty(n3, dtype=torch.float64, device="cuda")
     out = {}
     for sd, name in enumerate("xyz"):
+        if tune:
+            api.autotune_mixed_f32(sd, dims)
         ref = d64.clone()
         api.trid_strided_batch(a64, b64, c64, ref, solvedim=sd, dims=dims)
         runs = {}
@@ -510,7 +512,7 @@ def main():
             del a, b, c, d0, d
             torch.cuda.empty_cache()
             res = bench_mixed(torch, api, n, ref, tune) if args.only == "mixed" else \
-                bench_mixed_f32(torch, api, n, ref)
+                bench_mixed_f32(torch, api, n, ref, tune)
         else:
             run = {"fp32": lambda: bench_fp32_dirs(torch, api, n, tune),
                    "adjoint": lambda: bench_adjoint(torch, api),
@@ -588,7 +590,7 @@ def main():
         log("fp32: %s" % extras["fp32_per_direction"])
         extras["fp32_refined_vs_fp64"] = bench_mixed(torch, api, n, per_dir_ms, tune)
         log("mixed: %s" % extras["fp32_refined_vs_fp64"])
-        extras["fp32_stored_refined_vs_fp64"] = bench_mixed_f32(torch, api, n, per_dir_ms)
+        extras["fp32_stored_refined_vs_fp64"] = bench_mixed_f32(torch, api, n, per_dir_ms, tune)
         log("mixed, FP32-stored: %s" % extras["fp32_stored_refined_vs_fp64"])
         extras["adjoint_256"] = bench_adjoint(torch, api)
         log("adjoint: %s" % extras["adjoint_256"])

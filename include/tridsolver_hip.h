@@ -87,6 +87,9 @@ EXTERN_C tridStatus_t tridHipAutotune(int fp64, int inc, int ndim, int solvedim,
 /* Tunes tridDmtsvStridedBatchMixed (2 refinements) for one geometry. */
 EXTERN_C tridStatus_t tridHipAutotuneMixed(int ndim, int solvedim, const int *dims,
                                            const int *pads, float *best_ms);
+/* Tunes tridDmtsvStridedBatchMixedF32 (1 refinement) for one geometry. */
+EXTERN_C tridStatus_t tridHipAutotuneMixedF32(int ndim, int solvedim, const int *dims,
+                                              const int *pads, float *best_ms);
 /* Tunes the ragged kernel (rows per lane, workgroups per CU) on the caller's
  * device offsets with scratch values; cached per (precision, max_len). */
 EXTERN_C tridStatus_t tridHipAutotuneRagged(int fp64, const int64_t *offsets, int64_t nsys,

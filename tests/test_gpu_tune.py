@@ -67,6 +67,23 @@ def test_autotune_on_first_use():
         _lib.lib().tridHipSetAutotuneMode(0)
 
 
+def test_autotune_mixed_f32():
+    """Tuning the FP32-stored solve picks a valid shape per direction (the
+    solve then reports it) and the result stays within 1e-12 of the
+    long-double oracle's solve of the promoted values."""
+    dims = [128, 64, 32]
+    rng = np.random.default_rng(21)
+    n = int(np.prod(dims))
+    a, b, c, d = H.diag_dominant(rng, n, np.float32)
+    t = [dev(v) for v in (a, b, c, d)]
+    for sd in range(3):
+        assert api.autotune_mixed_f32(sd, dims) > 0
+        x = api.mixed_f32_strided_batch(*t, solvedim=sd, dims=dims, refinements=1, sync=True)
+        assert api.last_config()[0] == 10
+        xl = oracle.batch_ld(a, b, c, d, ndim=3, solvedim=sd, dims=dims)
+        assert H.norm_rel_err(x.cpu().numpy(), xl) <= 1e-12
+
+
 def test_autotune_mixed_and_ragged():
     dims = [128, 64, 32]
     for sd in range(3):

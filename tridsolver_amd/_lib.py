@@ -84,6 +84,7 @@ SIGNATURES = {
     "tridHipSetAutotuneMode": ([I], I),
     "tridHipAutotune": ([I, I, I, I, P, P, C.POINTER(C.c_float)], I),
     "tridHipAutotuneMixed": ([I, I, P, P, C.POINTER(C.c_float)], I),
+    "tridHipAutotuneMixedF32": ([I, I, P, P, C.POINTER(C.c_float)], I),
     "tridHipAutotuneRagged": ([I, P, L, L], I),
     "tridHipAutotuneSave": ([C.c_char_p], I),
     "tridHipAutotuneLoad": ([C.c_char_p], I),

@@ -336,6 +336,16 @@ def autotune_mixed(solvedim, dims, pads=None):
     return ms.value
 
 
+def autotune_mixed_f32(solvedim, dims, pads=None):
+    """Tunes the FP32-stored solve (one refinement) for one geometry; returns
+    the best time in ms."""
+    pads = dims if pads is None else pads
+    ms = C.c_float(0)
+    check(_lib.lib().tridHipAutotuneMixedF32(len(dims), solvedim, int3(dims), int3(pads),
+                                             C.byref(ms)), "tridHipAutotuneMixedF32")
+    return ms.value
+
+
 def autotune_ragged(dtype, offsets, max_len):
     check(_lib.lib().tridHipAutotuneRagged(1 if dtype == torch.float64 else 0, _ptr(offsets),
                                            offsets.numel() - 1, int(max_len)),

@@ -184,7 +184,7 @@ EXTERN_C tridStatus_t tridDmtsvStridedBatchMixedF32(const TridParams *ctx, const
   if (forced.M != 0) {
     cfg = forced;
     if (!mixed32_cfg_valid(g, cfg)) return TRID_STATUS_INVALID_VALUE;
-  } else if (!mixed32_default_cfg(g, refinements, &cfg)) {
+  } else if (!mixed32_tuned_config(dev, g, refinements, &cfg)) {
     return TRID_STATUS_INVALID_VALUE;
   }
   record_config(10, &cfg);

@@ -13,6 +13,9 @@ bool tuned_config(int dev, const Geom &g, HybridCfg *cfg);
 // Shape of the fused FP32/FP64-refinement kernel for g.
 bool mixed_tuned_config(int dev, const Geom &g, HybridCfg *cfg);
 
+// Shape of the FP32-stored solve for g at `refine` refinements.
+bool mixed32_tuned_config(int dev, const Geom &g, int refine, HybridCfg *cfg);
+
 // Rows per lane M, workgroups per CU and solve launches (1, or 2 with the
 // short systems apart) of the ragged persistent kernel for systems up to
 // max_len long.

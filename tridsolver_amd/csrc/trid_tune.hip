@@ -35,7 +35,7 @@
 namespace trid {
 namespace {
 
-enum Kind : int { KIND_SOLVE = 0, KIND_MIXED = 1, KIND_RAGGED = 2 };
+enum Kind : int { KIND_SOLVE = 0, KIND_MIXED = 1, KIND_RAGGED = 2, KIND_MIXED32 = 3 };
 
 struct Key {
   int dev, kind, fp64, inc;
@@ -48,7 +48,7 @@ struct Key {
 
 struct Choice {
   int algo;  // 1 lane, 3 hybrid (solve); 2 mixed; 5 ragged (cfg.M, cfg.G = per_cu,
-             // cfg.V = solve launches)
+             // cfg.V = solve launches); 10 FP32-stored mixed
   HybridCfg cfg;
   float ms;
 };
@@ -321,6 +321,73 @@ tridStatus_t tune_mixed(int dev, const Geom &g, Choice *best) {
   return best->algo ? TRID_STATUS_SUCCESS : TRID_STATUS_EXECUTION_FAILED;
 }
 
+// The FP32-stored solve (tridDmtsvStridedBatchMixedF32) at one refinement:
+// contiguous systems the register kernel over M x workgroup size x G;
+// strided ones the wide kernel (128- and 64-byte rows) over G, and the
+// register kernel at 16 and 8 rows per lane.
+tridStatus_t tune_mixed32(int dev, const Geom &g, Choice *best) {
+  (void)dev;
+  hipStream_t st = nullptr;
+  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess)
+    return TRID_STATUS_EXECUTION_FAILED;
+  const size_t n = extent_of(g);
+  Scratch s;
+  if (hipMalloc(&s.mem, sizeof(float) * n * 4 + sizeof(double) * n + 256) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipStreamDestroy(st);
+    return TRID_STATUS_ALLOC_FAILED;
+  }
+  float *a = static_cast<float *>(s.mem), *b = a + n, *c = b + n, *d = c + n;
+  double *x = reinterpret_cast<double *>(
+      (reinterpret_cast<uintptr_t>(d + n) + 255) & ~uintptr_t(255));
+  hipLaunchKernelGGL(tune_fill<float>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, b,
+                     c, d, (float *)nullptr, n);
+  best->algo = 0;
+  best->ms = INFINITY;
+  std::vector<HybridCfg> cands;
+  static const int Gs[] = {1, 8, 16, 32};
+  if (g.stride == 1) {
+    for (int M : {8, 16})
+      for (int W : {1, 2, 4, 8, 16})
+        for (int G : {1, 8}) {
+          HybridCfg cfg;
+          cfg.M = M; cfg.P = (int)((g.N + M - 1) / M); cfg.W = W; cfg.G = G; cfg.V = 0;
+          if ((int64_t)W > g.n0 && W > 1) continue;
+          cands.push_back(cfg);
+        }
+  } else {
+    const int wide[][2] = {{32, 32}, {16, 32}, {32, 16}};  // {M, W}
+    for (const auto &mw : wide)
+      for (int G : Gs) {
+        HybridCfg cfg;
+        cfg.M = mw[0]; cfg.W = mw[1]; cfg.G = G; cfg.V = 3;
+        const int per_wave = 64 / cfg.W;
+        cfg.P = (int)((g.N + cfg.M - 1) / cfg.M);
+        cfg.P = (cfg.P + per_wave - 1) / per_wave * per_wave;
+        cands.push_back(cfg);
+      }
+    for (int M : {8, 16})
+      for (int W : {4, 8, 16}) {
+        HybridCfg cfg;
+        cfg.M = M; cfg.P = (int)((g.N + M - 1) / M); cfg.W = W; cfg.G = 8; cfg.V = 0;
+        cands.push_back(cfg);
+      }
+  }
+  for (const HybridCfg &cfg : cands) {
+    if (!mixed32_cfg_valid(g, cfg)) continue;
+    const float ms =
+        time_on(st, [&] { return launch_mixed32(g, cfg, 1, a, b, c, d, x, st); });
+    if (ms < best->ms) {
+      best->ms = ms;
+      best->algo = 10;
+      best->cfg = cfg;
+    }
+  }
+  (void)hipStreamSynchronize(st);
+  (void)hipStreamDestroy(st);
+  return best->algo ? TRID_STATUS_SUCCESS : TRID_STATUS_EXECUTION_FAILED;
+}
+
 }  // namespace
 
 template <typename REAL, bool INC>
@@ -355,6 +422,26 @@ bool mixed_tuned_config(int dev, const Geom &g, HybridCfg *cfg) {
     return true;
   }
   return mixed_default_cfg(g, cfg);
+}
+
+// Tuned shapes are timed at one refinement; more refinements keep the
+// defaults, which avoid the shapes that spill there.
+bool mixed32_tuned_config(int dev, const Geom &g, int refine, HybridCfg *cfg) {
+  if (refine <= 1) {
+    const Key k = key_of(dev, KIND_MIXED32, 0, 0, g);
+    Choice c;
+    if (!lookup(k, &c) && autotune_mode() == 1 && !capturing(current_stream())) {
+      if (tune_mixed32(dev, g, &c) == TRID_STATUS_SUCCESS) store(k, c);
+      else c.algo = 0;
+    } else if (!lookup(k, &c)) {
+      c.algo = 0;
+    }
+    if (c.algo == 10 && mixed32_cfg_valid(g, c.cfg)) {
+      *cfg = c.cfg;
+      return true;
+    }
+  }
+  return mixed32_default_cfg(g, refine, cfg);
 }
 
 template <typename REAL>
@@ -478,6 +565,22 @@ EXTERN_C tridStatus_t tridHipAutotuneMixed(int ndim, int solvedim, const int *di
   Choice c;
   if ((s = tune_mixed(dev, g, &c)) != TRID_STATUS_SUCCESS) return s;
   store(key_of(dev, KIND_MIXED, 1, 0, g), c);
+  if (best_ms) *best_ms = c.ms;
+  return TRID_STATUS_SUCCESS;
+}
+
+EXTERN_C tridStatus_t tridHipAutotuneMixedF32(int ndim, int solvedim, const int *dims,
+                                              const int *pads, float *best_ms) {
+  Geom g;
+  bool empty = false;
+  tridStatus_t s = make_geom(ndim, solvedim, dims, pads, &g, &empty);
+  if (s != TRID_STATUS_SUCCESS) return s;
+  int dev = 0;
+  if ((s = check_device(&dev)) != TRID_STATUS_SUCCESS) return s;
+  if (empty) return TRID_STATUS_SUCCESS;
+  Choice c;
+  if ((s = tune_mixed32(dev, g, &c)) != TRID_STATUS_SUCCESS) return s;
+  store(key_of(dev, KIND_MIXED32, 0, 0, g), c);
   if (best_ms) *best_ms = c.ms;
   return TRID_STATUS_SUCCESS;
 }
