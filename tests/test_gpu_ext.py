@@ -502,6 +502,44 @@ def test_ragged_too_long_is_reported():
         api.ragged_batch(dev(a), dev(b), dev(c), td, dev(off), max_len=100000, sync=True)
 
 
+def test_mixed_f32_graph_replay():
+    """The FP32-stored solves (strided along z, and ragged with its workspace
+    reserved first) captured once into one hipGraph and replayed several times
+    with eager work in between: every replay equals the eager results."""
+    rng = np.random.default_rng(17)
+    dims = [64, 40, 96]
+    a, b, c, d = (dev(v) for v in _f32_case(rng, int(np.prod(dims))))
+    lens = np.round(np.exp(rng.uniform(np.log(8), np.log(4096), 300))).astype(np.int64)
+    off, ra, rb, rc, rd = ragged_case(rng, lens, np.float32)
+    r = [dev(v) for v in (ra, rb, rc, rd)]
+    toff = dev(off)
+    eager_s = api.mixed_f32_strided_batch(a, b, c, d, solvedim=2, dims=dims, sync=True)
+    eager_r = api.ragged_batch_mixed_f32(*r, toff, max_len=4096, sync=True)
+    s = torch.cuda.Stream()
+    xs = torch.empty_like(eager_s)
+    xr = torch.empty_like(eager_r)
+    with torch.cuda.stream(s):
+        assert api._lib.lib().tridHipSetStream(api.C.c_void_p(s.cuda_stream)) == 0
+        assert api._lib.lib().tridHipReserveWorkspace(
+            api._lib.lib().tridHipRaggedWorkspaceBytes(len(lens))) == 0
+        api.mixed_f32_strided_batch(a, b, c, d, xs, solvedim=2, dims=dims, stream=s)
+        api.ragged_batch_mixed_f32(*r, toff, xr, max_len=4096, stream=s)  # warm-up
+    s.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        api.mixed_f32_strided_batch(a, b, c, d, xs, solvedim=2, dims=dims, stream=s)
+        api.ragged_batch_mixed_f32(*r, toff, xr, max_len=4096, stream=s)
+    for k in range(4):
+        xs.zero_()
+        xr.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(xs, eager_s) and torch.equal(xr, eager_r), k
+        d.mul_(1.0)  # eager work between replays
+    api.use_torch_stream(torch.cuda.default_stream())
+
+
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_ragged_graph_replay(dtype):
     """Capture the planning kernels and the persistent solve once; replay it
