@@ -277,10 +277,14 @@ bool adi_cfg(int dev, const Geom &g, HybridCfg *cfg) {
   if (!tuned_config<REAL, INC>(dev, g, cfg)) return false;
   if (cfg->V != 0) {
     // a variant shape: keep its chunking (M, P: same arithmetic) with plain
-    // 128-byte rows, which suit a sweep that reads only d
+    // loads of 16 systems per workgroup (128-byte FP64 rows, 64-byte FP32
+    // rows), which suit a sweep that reads only d. Measured on the 256^3
+    // matrix-free step over W in {8, 16, 32, 64} and G in {1, 8, 16}: FP32
+    // 0.123 ms at W = 16 against 0.134 at 32 (128-byte rows) and 0.143 at 64;
+    // FP64 0.198-0.200 ms at W = 16, 0.205 at 8, 0.24 at 32.
     HybridCfg c = *cfg;
     c.V = 0;
-    c.W = 128 / (int)sizeof(REAL);
+    c.W = 16;
     if (hybrid_cfg_valid<REAL>(g, c)) {
       *cfg = c;
       return true;
