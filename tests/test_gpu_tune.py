@@ -84,6 +84,31 @@ def test_autotune_mixed_f32():
         assert H.norm_rel_err(x.cpu().numpy(), xl) <= 1e-12
 
 
+def test_autotune_mixed_f32_save_load(tmp_path):
+    """The FP32-stored solve's tuned shapes survive save, clear and load: the
+    solve after loading reports the shape it reported after tuning."""
+    dims = [96, 48, 40]
+    rng = np.random.default_rng(22)
+    a, b, c, d = H.diag_dominant(rng, int(np.prod(dims)), np.float32)
+    t = [dev(v) for v in (a, b, c, d)]
+    _lib.lib().tridHipAutotuneClear()
+    tuned = {}
+    for sd in range(3):
+        assert api.autotune_mixed_f32(sd, dims) > 0
+        api.mixed_f32_strided_batch(*t, solvedim=sd, dims=dims, refinements=1, sync=True)
+        tuned[sd] = api.last_config()
+    path = str(tmp_path / "tune_f32.txt").encode()
+    assert _lib.lib().tridHipAutotuneSave(path) == 0
+    assert _lib.lib().tridHipAutotuneClear() == 0
+    assert _lib.lib().tridHipAutotuneLoad(path) == 0
+    for sd in range(3):
+        x = api.mixed_f32_strided_batch(*t, solvedim=sd, dims=dims, refinements=1, sync=True)
+        assert api.last_config() == tuned[sd]
+        xl = oracle.batch_ld(a, b, c, d, ndim=3, solvedim=sd, dims=dims)
+        assert H.norm_rel_err(x.cpu().numpy(), xl) <= 1e-12
+    _lib.lib().tridHipAutotuneClear()
+
+
 def test_autotune_mixed_and_ragged():
     dims = [128, 64, 32]
     for sd in range(3):
