@@ -109,6 +109,35 @@ def test_autotune_mixed_f32_save_load(tmp_path):
     _lib.lib().tridHipAutotuneClear()
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_autotune_ragged_save_load(dtype, tmp_path):
+    """A tuned ragged shape (rows per lane, workgroups per CU) is cached per
+    precision and maximum length, and survives save, clear and load."""
+    rng = np.random.default_rng(23)
+    lens = np.round(np.exp(rng.uniform(np.log(8), np.log(2048), 1500))).astype(np.int64)
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    a, b, c, d = H.diag_dominant(rng, int(off[-1]), dtype)
+    toff = dev(off)
+    tdt = torch.float64 if dtype == np.float64 else torch.float32
+    want = d.copy()
+    oracle.ragged(a, b, c, want, off)
+    _lib.lib().tridHipAutotuneClear()
+    api.autotune_ragged(tdt, toff, 2048)
+    td = dev(d)
+    api.ragged_batch(dev(a), dev(b), dev(c), td, toff, max_len=2048, sync=True)
+    tuned = api.last_config()
+    H.assert_norm_close(td.cpu().numpy(), want, dtype)
+    path = str(tmp_path / "tune_ragged.txt").encode()
+    assert _lib.lib().tridHipAutotuneSave(path) == 0
+    assert _lib.lib().tridHipAutotuneClear() == 0
+    assert _lib.lib().tridHipAutotuneLoad(path) == 0
+    td = dev(d)
+    api.ragged_batch(dev(a), dev(b), dev(c), td, toff, max_len=2048, sync=True)
+    assert api.last_config() == tuned
+    H.assert_norm_close(td.cpu().numpy(), want, dtype)
+    _lib.lib().tridHipAutotuneClear()
+
+
 def test_autotune_mixed_and_ragged():
     dims = [128, 64, 32]
     for sd in range(3):
