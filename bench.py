@@ -53,7 +53,7 @@ def parse():
                     help="z-extent of the CPU baseline sample (n x n x slab)")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--only", choices=["fp32", "mixed", "mixed-f32", "adjoint", "ragged", "adi",
-                                       "partitioned", "cpu-extras"],
+                                       "partitioned", "cpu-extras", "oracle-checks"],
                     help="run one secondary workload and print its JSON instead of the bench "
                          "line (for profiling)")
     return ap.parse_args()
@@ -242,6 +242,83 @@ def cpu_extras(reps=3):
                                        nthreads=threads))
         per[name] = round(m / t / 1e9, 3)
     out["adjoint_fp64_all_grads_gpts_s"] = dict(per, kind="port", sample="256x256x64 FP64")
+    return out
+
+
+def oracle_checks(torch, api, n, samp=64):
+    """Part of the CPU-baseline leg: the long-double oracle checks the
+    mixed-precision results the line reports, on a sample of systems. The
+    same seeded inputs as bench_mixed / bench_mixed_f32 / bench_ragged are
+    solved on the GPU at one refinement with the shapes tuned there; then, per
+    direction, an 8 x samp block of whole systems (all n rows along the solve
+    direction) is copied to the host and compared with the oracle's
+    long-double solve of those systems (normwise max|x - x_ld| / max|x_ld|).
+    The ragged batches are checked on their first 2,000 systems."""
+    import numpy as np
+
+    import oracle
+    out = {}
+    dims = [n, n, n]
+
+    def block(t, sd):
+        v = t.view(n, n, n)  # (z, y, x)
+        if sd == 0:
+            return v[0:8, 0:samp, :].contiguous().cpu().numpy(), [n, samp, 8]
+        if sd == 1:
+            return v[0:8, :, 0:samp].contiguous().cpu().numpy(), [samp, n, 8]
+        return v[:, 0:8, 0:samp].contiguous().cpu().numpy(), [samp, 8, n]
+
+    a, b, c, d0 = gen_gpu(torch, n ** 3, torch.float64, seed=64)
+    per = {}
+    for sd, name in enumerate("xyz"):
+        x = d0.clone()
+        api.mixed_strided_batch(a, b, c, x, solvedim=sd, dims=dims, refinements=1, sync=True)
+        blk = [block(t, sd) for t in (a, b, c, d0, x)]
+        sdims = blk[0][1]
+        xl = oracle.batch_ld(*(t[0].ravel() for t in blk[:4]), ndim=3, solvedim=sd, dims=sdims)
+        got = blk[4][0].ravel()
+        per[name] = float(np.abs(got - xl).max() / np.abs(xl).max())
+        del x
+    out["fp64_stored_1_refinement"] = dict(per, sample="8 x %d systems of %d rows per direction"
+                                           % (samp, n))
+    del a, b, c, d0
+    a, b, c, d = gen_gpu(torch, n ** 3, torch.float32, seed=32)
+    xs = torch.empty(n ** 3, dtype=torch.float64, device="cuda")
+    per = {}
+    for sd, name in enumerate("xyz"):
+        api.mixed_f32_strided_batch(a, b, c, d, xs, solvedim=sd, dims=dims, refinements=1,
+                                    sync=True)
+        blk = [block(t, sd) for t in (a, b, c, d, xs)]
+        xl = oracle.batch_ld(*(t[0].ravel().astype(np.float64) for t in blk[:4]), ndim=3,
+                             solvedim=sd, dims=blk[0][1])
+        got = blk[4][0].ravel()
+        per[name] = float(np.abs(got - xl).max() / np.abs(xl).max())
+    out["fp32_stored_1_refinement"] = dict(
+        per, sample="8 x %d systems of %d rows per direction; oracle solves the FP32 values "
+                    "promoted to FP64" % (samp, n))
+    del a, b, c, d, xs
+    rng = np.random.default_rng(40000)
+    lens = np.round(np.exp(rng.uniform(np.log(8), np.log(8192), 40000))).astype(np.int64)
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    total = int(off[-1])
+    toff = torch.from_numpy(off).cuda()
+    k = 2000
+    m = int(off[k])
+    a, b, c, d = gen_gpu(torch, total, torch.float64, seed=8192)
+    x = d.clone()
+    api.ragged_batch_mixed(a, b, c, x, toff, max_len=8192, refinements=1, sync=True)
+    host = [t[:m].cpu().numpy() for t in (a, b, c, d, x)]
+    xl = oracle.ragged_ld(*host[:4], off[:k + 1])
+    out["ragged_fp64_stored_1_refinement"] = float(np.abs(host[4] - xl).max() / np.abs(xl).max())
+    del a, b, c, d, x
+    a, b, c, d = gen_gpu(torch, total, torch.float32, seed=8192)
+    x = api.ragged_batch_mixed_f32(a, b, c, d, toff, max_len=8192, refinements=1, sync=True)
+    host = [t[:m].cpu().numpy().astype(np.float64) for t in (a, b, c, d, x)]
+    xl = oracle.ragged_ld(*host[:4], off[:k + 1])
+    out["ragged_fp32_stored_1_refinement"] = float(np.abs(host[4] - xl).max() / np.abs(xl).max())
+    out["ragged_sample"] = "first %d of the 40,000 systems (%d unknowns)" % (k, m)
+    del a, b, c, d, x
+    torch.cuda.empty_cache()
     return out
 
 
@@ -519,7 +596,8 @@ def main():
                    "ragged": lambda: bench_ragged(torch, api, tune),
                    "adi": lambda: bench_adi(torch),
                    "partitioned": lambda: bench_partitioned(torch, api),
-                   "cpu-extras": cpu_extras}[args.only]
+                   "cpu-extras": cpu_extras,
+                   "oracle-checks": lambda: oracle_checks(torch, api, n)}[args.only]
             res = run()
         print(json.dumps({"only": args.only, "result": res}))
         return
@@ -608,6 +686,18 @@ def main():
         if not args.quick:
             extras["cpu_other_workloads"] = cpu_extras()
             log("cpu extras: %s" % extras["cpu_other_workloads"])
+            chk = oracle_checks(torch, api, n)
+            log("oracle checks: %s" % chk)
+            extras["mixed_max_rel_err_vs_long_double_oracle"] = chk
+            for key, src in (("fp32_refined_vs_fp64", "fp64_stored_1_refinement"),
+                             ("fp32_stored_refined_vs_fp64", "fp32_stored_1_refinement")):
+                for name in "xyz":
+                    extras[key][name]["max_rel_err_vs_oracle_1_refinement_sample"] = \
+                        chk[src][name]
+            extras["ragged_40k"]["mixed"]["max_rel_err_vs_oracle_1_refinement_sample"] = \
+                chk["ragged_fp64_stored_1_refinement"]
+            extras["ragged_40k"]["mixed_f32_stored"][
+                "max_rel_err_vs_oracle_1_refinement_sample"] = chk["ragged_fp32_stored_1_refinement"]
 
     out = {
         "metric": "grid pts solved/s (FP64 512^3 x/y/z batched tridiagonal solves)",

@@ -64,6 +64,8 @@ def _load_oracle():
         f = getattr(lib, name)
         f.argtypes = [_P, _P, _P, _P, _P, _P, _L, _I, _I]
         f.restype = None
+    lib.orc_ragged_ld.argtypes = [_P, _P, _P, _P, _P, _P, _L, _I]
+    lib.orc_ragged_ld.restype = None
     lib.orc_adjoint_d.argtypes = [_P] * 9 + [_I, _I, _P, _P, _I]
     lib.orc_adjoint_d.restype = _I
     lib.orc_adi_preproc_d.argtypes = [_D, _P, _P, _P, _P, _P, _I, _I, _I]
@@ -143,6 +145,17 @@ def ragged(a, b, c, d, offsets, u=None, *, inc=False, nthreads=0):
     f = lib().orc_ragged_d if d.dtype == np.float64 else lib().orc_ragged_s
     f(_ptr(a), _ptr(b), _ptr(c), _ptr(d), _ptr(u), _ptr(offsets), len(offsets) - 1,
       int(bool(inc)), nthreads)
+
+
+def ragged_ld(a, b, c, d, offsets, nthreads=0):
+    """Long-double solve of each FP64 system of a ragged batch, rounded to
+    FP64 (error reference); entries no system owns stay 0."""
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    arrs = [np.ascontiguousarray(t, dtype=np.float64) for t in (a, b, c, d)]
+    x = np.zeros_like(arrs[3])
+    lib().orc_ragged_ld(*[_ptr(t) for t in arrs], _ptr(x), _ptr(offsets), len(offsets) - 1,
+                        nthreads)
+    return x
 
 
 def adjoint(a, b, c, x, w, *, ndim, solvedim, dims, pads=None, nthreads=0):

@@ -25,6 +25,8 @@
  *   orc_adjoint_d     the identity <w, A^-1 d> = <A^-T w, d>, central finite
  *                     differences, and a dense transpose solve.
  *   orc_ragged_d      per-system dense LU and orc_thomas_d on each slice.
+ *   orc_ragged_s      per-system dense LU (FP32 tolerance), bit-exact
+ *                     against orc_batch_s on each slice, and the Inc form.
  *   orc_*_ld          long-double solve; used only to report errors. Pinned
  *                     by the dense-LU brute force on small systems.
  */
@@ -290,6 +292,30 @@ void orc_ragged_s(const float *a, const float *b, const float *c, float *d,
       if (L > 0)
         thomas_s(a + o, b + o, c + o, d + o, u ? u + o : 0, L, 1, inc, 0, s2,
                  s2 + maxlen);
+    }
+    free(s2);
+  }
+}
+
+/* Long-double error reference of a ragged FP64 batch (x = A^-1 d per      */
+/* system, rounded to double); empty systems are skipped.                   */
+void orc_ragged_ld(const double *a, const double *b, const double *c,
+                   const double *d, double *x, const int64_t *offsets,
+                   int64_t nsys, int nthreads) {
+  int64_t maxlen = 1;
+  for (int64_t s = 0; s < nsys; s++) {
+    int64_t L = offsets[s + 1] - offsets[s];
+    if (L > maxlen) maxlen = L;
+  }
+  int nt = nthreads > 0 ? nthreads : omp_max_threads_();
+#pragma omp parallel num_threads(nt)
+  {
+    ldouble *s2 = (ldouble *)malloc(sizeof(ldouble) * 2 * (size_t)maxlen);
+#pragma omp for schedule(dynamic, 16)
+    for (int64_t s = 0; s < nsys; s++) {
+      int64_t o = offsets[s], L = offsets[s + 1] - o;
+      if (L > 0)
+        thomas_ld(a + o, b + o, c + o, d + o, x + o, L, 1, s2, s2 + maxlen);
     }
     free(s2);
   }

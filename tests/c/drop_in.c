@@ -1,8 +1,10 @@
 /*
  * drop_in.c - a C caller of libtridhip.so written the way the reference's
  * own C/CUDA callers use the library (apps/adi/src/adi_cuda.cu:294-323,
- * scalar/test/test_cuda.cu:21-34): device arrays from the HIP runtime, the
- * reference's tridsolver.h, the four solver symbols, ctx with opts and sync.
+ * scalar/test/test_cuda.cu:21-34): device arrays from the HIP runtime, this
+ * library's include/tridsolver.h (which declares the reference's four solver
+ * symbols and types), ctx with opts and sync. The same file also compiles as
+ * C++ against the reference's own scalar/include headers with FPPREC=1.
  *
  * It solves a diagonally dominant 40 x 36 x 33 FP64 batch along x, y and z
  * (plain, plain, Inc: the ADI sweep pattern), an FP32 solve along y, and the

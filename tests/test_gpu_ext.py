@@ -2,7 +2,7 @@
 (and its autograd wrapper), and ragged batches in one persistent launch.
 
 Tolerances (normwise, max|err| / max|ref|):
-  mixed, 2 refinements, vs the long-double oracle  <= 1e-12
+  mixed, 1 or 2 refinements, vs the long-double oracle  <= 1e-12
   mixed, 0 refinements (pure FP32 sweeps)          <= 1e-5
   adjoint FP64 vs the oracle's FP64 adjoint        <= 1e-12 (each output)
   adjoint FP32 vs the same                          <= 1e-5
@@ -51,9 +51,49 @@ def test_mixed_refinement_accuracy(dims):
                                     refinements=refine, sync=True)
             errs.append(H.norm_rel_err(td.cpu().numpy(), xl))
         assert errs[0] <= 1e-5, errs
+        assert errs[1] <= 1e-12, errs
         assert errs[2] <= 1e-12, errs
         assert errs[1] < errs[0]
         assert api.last_config()[0] == 2
+
+
+# The 512^3 row length along each direction, at a fraction of the grid; the
+# shapes the autotuner picked for the mixed solve at 512^3 (M = 16 rows per
+# lane, P = 32 lanes, W systems per workgroup, G, variant 3 = a and c in LDS).
+N512_CASES = [(0, [512, 40, 6], (16, 32, 4, 8, 3)),
+              (1, [48, 512, 6], (16, 32, 8, 16, 3)),
+              (2, [48, 6, 512], (16, 32, 16, 16, 3))]
+
+
+@pytest.mark.parametrize("sd,dims,tuned", N512_CASES)
+def test_mixed_one_refinement_n512(sd, dims, tuned):
+    """One refinement -- the setting the benchmark reports -- against the
+    long-double oracle at N = 512 along x, y and z: with the default shape,
+    with the shape tuned at 512^3 forced, and with the shape the autotuner
+    picks for this geometry. Each within 1e-12."""
+    rng = np.random.default_rng(512 + sd)
+    n = int(np.prod(dims))
+    a, b, c, d = H.diag_dominant(rng, n)
+    xl = oracle.batch_ld(a, b, c, d, ndim=3, solvedim=sd, dims=dims)
+    ta, tb, tc = dev(a), dev(b), dev(c)
+    errs = {}
+
+    def run(tag):
+        td = dev(d)
+        api.mixed_strided_batch(ta, tb, tc, td, solvedim=sd, dims=dims, refinements=1,
+                                sync=True)
+        errs[tag] = (H.norm_rel_err(td.cpu().numpy(), xl), api.last_config())
+
+    run("default")
+    with api.forced_hybrid(*tuned):
+        run("tuned at 512^3")
+        assert errs["tuned at 512^3"][1][1:4] == tuned[:3]
+    api.autotune_mixed(sd, dims)
+    run("autotuned here")
+    print("mixed, 1 refinement, solvedim %d dims %s vs long-double oracle: %s"
+          % (sd, dims, errs))
+    for tag, (err, cfg) in errs.items():
+        assert err <= 1e-12, (tag, err, cfg)
 
 
 def test_mixed_padded_layout():
@@ -626,6 +666,11 @@ def test_ragged_mixed_vs_oracle(M, launches):
                                refinements=2, sync=True)
         assert api.last_config()[:4:3] == (9, launches)
         check_ragged(off, a, b, c, d, td.cpu().numpy(), np.float64)
+        t1 = dev(d)  # one refinement: the setting the benchmark reports
+        api.ragged_batch_mixed(dev(a), dev(b), dev(c), t1, dev(off), max_len=max_len,
+                               refinements=1, sync=True)
+        xl = oracle.ragged_ld(a, b, c, d, off)
+        assert H.norm_rel_err(t1.cpu().numpy(), xl) <= 1e-12
         t0 = dev(d)
         api.ragged_batch_mixed(dev(a), dev(b), dev(c), t0, dev(off), max_len=max_len,
                                refinements=0, sync=True)

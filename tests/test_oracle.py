@@ -234,6 +234,55 @@ def test_oracle_ragged_matches_dense():
     np.testing.assert_allclose(u - u0, x, rtol=1e-12, atol=1e-14)
 
 
+def test_oracle_ragged_s_matches_dense_and_batch():
+    """orc_ragged_s: each FP32 system against a dense FP64 solve of the same
+    values (FP32 tolerance), bit for bit against orc_batch_s run on that slice
+    alone (the same trid_scalar arithmetic), and Inc = plain + u."""
+    rng = np.random.default_rng(18)
+    lens = [1, 2, 7, 64, 0, 3, 200, 33]
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    a, b, c, d = H.diag_dominant(rng, int(off[-1]), np.float32)
+    x = d.copy()
+    oracle.ragged(a, b, c, x, off)
+    for s in range(len(lens)):
+        sl = slice(off[s], off[s + 1])
+        if lens[s] == 0:
+            continue
+        want = H.dense_solve(*(v[sl].astype(np.float64) for v in (a, b, c, d)))
+        assert H.norm_rel_err(x[sl], want) < 1e-5, s
+        one = d[sl].copy()
+        oracle.batch(a[sl].copy(), b[sl].copy(), c[sl].copy(), one, ndim=1, solvedim=0,
+                     dims=[lens[s]])
+        assert np.array_equal(one, x[sl]), s
+    u0 = rng.uniform(-1, 1, int(off[-1])).astype(np.float32)
+    u = u0.copy()
+    dd = d.copy()
+    oracle.ragged(a, b, c, dd, off, u, inc=True)
+    assert np.array_equal(dd, d)
+    np.testing.assert_allclose(u - u0, x, rtol=1e-5, atol=1e-6)
+
+
+def test_oracle_ragged_ld_matches_dense():
+    """orc_ragged_ld (the long-double error reference of ragged batches)
+    against dense FP64 LU per system; entries past the last system stay 0."""
+    rng = np.random.default_rng(19)
+    lens = [1, 2, 5, 64, 0, 130]
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    a, b, c, d = H.diag_dominant(rng, int(off[-1]) + 3)
+    x = oracle.ragged_ld(a, b, c, d, off)
+    for s in range(len(lens)):
+        sl = slice(off[s], off[s + 1])
+        if lens[s]:
+            assert H.norm_rel_err(x[sl], H.dense_solve(a[sl], b[sl], c[sl], d[sl])) < 1e-14
+    assert np.all(x[off[-1]:] == 0)
+    # a dropped sub-diagonal term in the oracle would show: compare against a
+    # dense solve of the system without it
+    sl = slice(off[3], off[4])
+    a0 = a[sl].copy()
+    a0[1:] = 0
+    assert H.norm_rel_err(x[sl], H.dense_solve(a0, b[sl], c[sl], d[sl])) > 1e-3
+
+
 def test_oracle_adjoint_identities():
     """A^T lam = w; <w, A^-1 d> = <lam, d>; gradients by central differences."""
     rng = np.random.default_rng(9)

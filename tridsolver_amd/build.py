@@ -60,23 +60,37 @@ def build(verbose: bool = False) -> str:
     return out
 
 
-def build_c_caller() -> str:
-    """Compiles tests/c/drop_in.c, a plain C99 caller of the library through the
-    reference's header (gcc, linked against libtridhip.so and the HIP runtime,
-    found at run time through rpath). Returns the binary's path."""
+def _c_program(src, out, defines=(), lang="c", include=None):
+    """Compiles one C test program against the library's headers (or another
+    include directory), linked against libtridhip.so and the HIP runtime,
+    both found at run time through rpath."""
     root = os.path.dirname(HERE)
-    src = os.path.join(root, "tests", "c", "drop_in.c")
-    out = os.path.join(root, "tests", "c", "drop_in")
     rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
-    cmd = [os.environ.get("CC", "gcc"), "-std=c99", "-O2", "-Wall",
-           "-I" + os.path.join(root, "include"), "-I" + os.path.join(rocm, "include"),
-           "-D__HIP_PLATFORM_AMD__", src, "-o", out, "-L" + LIBDIR, "-ltridhip",
-           "-L" + os.path.join(rocm, "lib"), "-lamdhip64", "-lm",
-           "-Wl,-rpath,$ORIGIN/../../tridsolver_amd/lib", "-Wl,-rpath," + os.path.join(rocm, "lib")]
+    if lang == "c":
+        head = [os.environ.get("CC", "gcc"), "-std=c99"]
+    else:
+        head = [os.environ.get("CXX", "g++"), "-x", "c++", "-std=c++14"]
+    cmd = head + ["-O2", "-Wall"] + ["-D" + d for d in defines] + [
+        "-I" + (include or os.path.join(root, "include")), "-I" + os.path.join(rocm, "include"),
+        "-D__HIP_PLATFORM_AMD__", src, "-x", "none", "-o", out, "-L" + LIBDIR, "-ltridhip",
+        "-L" + os.path.join(rocm, "lib"), "-lamdhip64", "-lm",
+        "-Wl,-rpath,$ORIGIN/../../tridsolver_amd/lib", "-Wl,-rpath," + os.path.join(rocm, "lib")]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError("C caller build failed:\n" + r.stdout + r.stderr)
+        raise RuntimeError("C caller build failed (%s):\n%s" % (" ".join(cmd), r.stdout + r.stderr))
     return out
+
+
+def build_c_caller() -> str:
+    """Compiles the plain-C callers under tests/c: drop_in.c (this library's
+    headers, C99) and fp_caller.c once per precision (FPPREC 0 and 1, the
+    reference app's compile-time switch). Returns drop_in's path."""
+    root = os.path.dirname(HERE)
+    cdir = os.path.join(root, "tests", "c")
+    for prec, name in ((0, "fp_caller_f32"), (1, "fp_caller_f64")):
+        _c_program(os.path.join(cdir, "fp_caller.c"), os.path.join(cdir, name),
+                   defines=("FPPREC=%d" % prec,))
+    return _c_program(os.path.join(cdir, "drop_in.c"), os.path.join(cdir, "drop_in"))
 
 
 if __name__ == "__main__":
