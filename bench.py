@@ -440,6 +440,34 @@ def bench_adjoint(torch, api, n=256):
     return out
 
 
+def ragged_graph_ms(torch, api, a, b, c, d, toff, reps=20):
+    """Mean time per replay of the ragged solve (plan + persistent launch)
+    captured once in a hipGraph on a side stream whose workspace is reserved
+    beforehand; events on that stream around `reps` back-to-back replays."""
+    s = torch.cuda.Stream()
+    try:
+        with torch.cuda.stream(s):
+            api.reserve_workspace(api.ragged_workspace_bytes(toff.numel() - 1), stream=s)
+            api.ragged_batch(a, b, c, d, toff, max_len=8192, stream=s)  # warm-up
+        s.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            api.ragged_batch(a, b, c, d, toff, max_len=8192, stream=s)
+        with torch.cuda.stream(s):
+            g.replay()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(s)
+            for _ in range(reps):
+                g.replay()
+            e1.record(s)
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        del g
+        return ms
+    finally:
+        api.use_torch_stream(torch.cuda.current_stream())
+
+
 def bench_ragged(torch, api, tune):
     import numpy as np
     rng = np.random.default_rng(40000)
@@ -457,6 +485,12 @@ def bench_ragged(torch, api, tune):
         out[key] = {"ms": round(ms, 4), "gpts_s": round(total / ms / 1e6, 2),
                     "alg_GBs": round(gbs, 1), "frac_peak": round(gbs / PEAK_HBM_GBS, 4),
                     "config": api.last_config()}
+        gms = ragged_graph_ms(torch, api, a, b, c, d, toff)
+        gbs = (bpp * total + 8 * (len(off))) / (gms * 1e-3) / 1e9
+        out[key]["graph"] = {"ms_per_replay": round(gms, 4),
+                             "gpts_s": round(total / gms / 1e6, 2), "alg_GBs": round(gbs, 1),
+                             "what": "planning kernels + persistent solve captured once in a "
+                                     "hipGraph, replayed back to back"}
         del a, b, c, d
     # FP32 sweeps + FP64 refinement on the same 40,000 FP64 systems, at 1 and
     # 2 refinements; the line reports the fewest that meet 1e-12 against the

@@ -95,6 +95,34 @@ def test_adi_graph_replay_matches_eager(dtype):
 
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("opts", [(0, 1, 1), (1, 2, 1)])
+def test_adi_graph_run_lane_path(dtype, opts):
+    """The reference app's -opty/-optz (and -optx) choices that select the
+    per-lane solve, which needs a workspace, in a hipGraph-captured run: the
+    library reserves the capture stream's workspace before capturing. The
+    replayed steps equal the eager steps and the oracle ADI."""
+    nx, ny, nz, iters = 40, 36, 32, 3
+    u0 = adi.initial_u(nx, ny, nz, dtype, "random", seed=16)
+    u_cpu = _np(u0).copy()
+    ws1 = adi.Workspace(nx, ny, nz, dtype)
+    ws2 = adi.Workspace(nx, ny, nz, dtype)
+    ue = u0.clone()
+    for _ in range(iters):
+        adi.step(1.0, ue, ws1, opts=opts)
+    ug = u0.clone()
+    adi.run(1.0, ug, ws2, iters, graph=True, opts=opts, sync=True)
+    torch.cuda.synchronize()
+    assert torch.equal(ue, ug)
+    oracle.adi(1.0, u_cpu, nx, ny, nz, iters)
+    H.assert_norm_close(_np(ug), u_cpu, tol=1e-12 if dtype == torch.float64 else 1e-5)
+    # a second captured run reuses nothing of the first one's stream
+    adi.run(1.0, ug, ws2, 1, graph=True, opts=opts, sync=True)
+    adi.step(1.0, ue, ws1, opts=opts)
+    torch.cuda.synchronize()
+    assert torch.equal(ue, ug)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 def test_config_adi_256_20_steps(dtype):
     """ADI 256^3, lambda = 1, u0 ~ U(0,1) inside, Dirichlet 0, 20 steps; the
     GPU run (hipGraph replay) against the oracle ADI on the CPU."""

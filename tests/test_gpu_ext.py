@@ -616,6 +616,43 @@ def test_ragged_graph_replay(dtype):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_config_ragged_40k_graph(dtype):
+    """The 40,000-system config as a hipGraph-captured repeat: planning
+    kernels and the persistent solve captured once, replayed with eager work
+    in between; every replay equals the eager solve bit for bit, and the
+    replayed result matches the oracle."""
+    rng = np.random.default_rng(40000)
+    lens = np.round(np.exp(rng.uniform(np.log(8), np.log(8192), 40000))).astype(np.int64)
+    off, a, b, c, d = ragged_case(rng, lens, dtype)
+    ta, tb, tc, toff, t_in = dev(a), dev(b), dev(c), dev(off), dev(d)
+    eager = t_in.clone()
+    api.ragged_batch(ta, tb, tc, eager, toff, max_len=8192, sync=True)
+    s = torch.cuda.Stream()
+    work = torch.empty_like(t_in)
+    with torch.cuda.stream(s):
+        api.reserve_workspace(api.ragged_workspace_bytes(len(lens)), stream=s)
+        work.copy_(t_in)
+        api.ragged_batch(ta, tb, tc, work, toff, max_len=8192, stream=s)  # warm-up
+    s.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        api.ragged_batch(ta, tb, tc, work, toff, max_len=8192, stream=s)
+    try:
+        for k in range(3):
+            work.copy_(t_in)
+            torch.cuda.synchronize()
+            g.replay()
+            torch.cuda.synchronize()
+            assert torch.equal(work, eager), k
+            t_in.mul_(1.0)  # eager work between replays
+    finally:
+        api.use_torch_stream(torch.cuda.default_stream())
+    err = check_ragged(off, a, b, c, d, work.cpu().numpy(), dtype)
+    print("ragged 40k %s graph replay: config %s, normwise rel err vs oracle %.2e"
+          % (np.dtype(dtype).name, api.last_config(), err))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_config_ragged_40k(dtype):
     """40,000 systems, lengths round(exp(U(ln 8, ln 8192))) (FP32: up to 8192
     with M = 8), diagonally dominant; one persistent launch vs the oracle."""

@@ -145,6 +145,20 @@ def part_solve(a, b, c, d, bnd_all, *, solvedim, shard, nshards, dims=None, pads
     return u if inc else d
 
 
+def reserve_workspace(nbytes, stream=None):
+    """Grows the library's workspace for (device, stream) to at least nbytes
+    (tridHipReserveWorkspace): done before a hipGraph capture, which cannot
+    allocate. Binds this thread's library calls to `stream` (default: the
+    current torch stream)."""
+    use_torch_stream(stream)
+    check(_lib.lib().tridHipReserveWorkspace(int(nbytes)), "tridHipReserveWorkspace")
+
+
+def ragged_workspace_bytes(nsys):
+    """Workspace bytes a ragged batch of nsys systems plans in."""
+    return int(_lib.lib().tridHipRaggedWorkspaceBytes(int(nsys)))
+
+
 def last_config():
     """(algorithm, M, P, W, G) of this thread's last solve (1 lane, 3 hybrid)."""
     v = [C.c_int(0) for _ in range(5)]

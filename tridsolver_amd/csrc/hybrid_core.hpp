@@ -11,6 +11,9 @@
 //                  one barrier per exchange is enough because a thread that
 //                  passed exchange k's barrier knows every thread finished
 //                  reading exchange k-1's buffer
+//   from_above /   lane exchanges of the reduced system: within one wave by
+//   from_below /   ds_bpermute (no LDS store, no barrier) when every system's
+//   from_both      lanes lie in one wave, else through the Ring
 //   forward_spike / backward_spike / reduced_solve / substitute: phases (i)-(iv)
 #pragma once
 
@@ -309,42 +312,133 @@ __device__ __forceinline__ void backward_spike(TR (&ra)[M], TR (&rc)[M], TR (&rd
   }
 }
 
+// ---- lane exchanges ----------------------------------------------------------
+// A system's P lanes lie in one wave when the workgroup is one wave, or when
+// the system's lanes are consecutive (pstep 1) and P divides 64 (systems then
+// start at multiples of P). Such systems exchange values with ds_bpermute:
+// wave-local, no LDS store and no workgroup barrier. The values moved are
+// the same as through LDS, so results do not depend on the path.
+__device__ __forceinline__ bool wave_local(const Lane &L) {
+  return L.T <= 64 || (L.pstep == 1 && L.P <= 64 && (L.P & (L.P - 1)) == 0);
+}
+
+// v of lane `src` of this wave. ds_bpermute moves 32 bits per lane and reads
+// the lane given by bits 7:2 of the byte address, so src is taken modulo 64;
+// callers discard values from lanes outside their system.
+template <typename X>
+__device__ __forceinline__ X wave_read(X v, int src) {
+  static_assert(sizeof(X) % 4 == 0, "32-bit words");
+  constexpr int NW = sizeof(X) / 4;
+  int w[NW];
+  __builtin_memcpy(w, &v, sizeof(X));
+#pragma unroll
+  for (int k = 0; k < NW; ++k) w[k] = __builtin_amdgcn_ds_bpermute(src << 2, w[k]);
+  X r;
+  __builtin_memcpy(&r, w, sizeof(X));
+  return r;
+}
+
+// vm[k] = v[k] of thread t - dm (0 unless okm) and vp[k] = v[k] of thread
+// t + dp (0 unless okp), for the K values of every thread of the workgroup.
+// Every thread calls it (it contains a barrier on the LDS path). SIDES: 1
+// below only, 2 above only, 3 both.
+template <int SIDES, int K, typename X>
+__device__ __forceinline__ void exchange(const Lane &L, Ring &ring, bool wave, const X (&v)[K],
+                                         int dm, bool okm, X (&vm)[K], int dp, bool okp,
+                                         X (&vp)[K]) {
+  if (wave) {
+    const int lane = L.t & 63;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if (SIDES & 1) {
+        const X m = wave_read(v[k], lane - dm);
+        vm[k] = okm ? m : X(0);
+      }
+      if (SIDES & 2) {
+        const X q = wave_read(v[k], lane + dp);
+        vp[k] = okp ? q : X(0);
+      }
+    }
+    return;
+  }
+  X *q = ring.next<X>();
+#pragma unroll
+  for (int k = 0; k < K; ++k) q[k * L.T + L.t] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if (SIDES & 1) vm[k] = X(0);
+    if (SIDES & 2) vp[k] = X(0);
+  }
+  if (SIDES & 1) {
+    if (okm) {
+      const int sl = L.t - dm;
+#pragma unroll
+      for (int k = 0; k < K; ++k) vm[k] = q[k * L.T + sl];
+    }
+  }
+  if (SIDES & 2) {
+    if (okp) {
+      const int sl = L.t + dp;
+#pragma unroll
+      for (int k = 0; k < K; ++k) vp[k] = q[k * L.T + sl];
+    }
+  }
+  ring.release();
+}
+
+// K values of chunk p+1 (0 for the last chunk).
+template <int K, typename X>
+__device__ __forceinline__ void from_above(const Lane &L, Ring &ring, bool wave,
+                                           const X (&v)[K], X (&vp)[K]) {
+  X unused[K];
+  exchange<2, K>(L, ring, wave, v, 0, false, unused, L.pstep, L.p + 1 < L.P, vp);
+}
+
+// K values of chunk p-1 (0 for the first chunk).
+template <int K, typename X>
+__device__ __forceinline__ void from_below(const Lane &L, Ring &ring, bool wave,
+                                           const X (&v)[K], X (&vm)[K]) {
+  X unused[K];
+  exchange<1, K>(L, ring, wave, v, L.pstep, L.p > 0, vm, 0, false, unused);
+}
+
+// K values of chunks p-h and p+h (0 outside the system): one PCR step.
+template <int K, typename X>
+__device__ __forceinline__ void from_both(const Lane &L, Ring &ring, bool wave, int h,
+                                          const X (&v)[K], X (&vm)[K], X (&vp)[K]) {
+  exchange<3, K>(L, ring, wave, v, h * L.pstep, L.p - h >= 0, vm, h * L.pstep, L.p + h < L.P,
+                 vp);
+}
+
 // (iii) reduced equation of this chunk from its last-row forward values and
 // chunk p+1's first row, then PCR over the P chunks of each system. Returns
 // y = x at this chunk's last row and ym = x at the previous chunk's last row.
-template <typename TR>
+// WAVE: the kernel's systems are contiguous (pstep 1), so they may lie in one
+// wave; strided kernels pass false and keep only the LDS path (the wave path
+// would add registers to kernels whose systems span waves).
+template <typename TR, bool WAVE = false>
 __device__ __forceinline__ void reduced_solve(const Lane &L, Ring &ring, TR al_e, TR ga_e,
                                               TR de_e, TR f_d, TR f_a, TR f_g, TR &y,
                                               TR &ym) {
-  const int T = L.T, t = L.t, p = L.p, P = L.P, ps = L.pstep;
-  TR *q = ring.next<TR>();
-  q[t] = f_d; q[T + t] = f_a; q[2 * T + t] = f_g;
-  __syncthreads();
-  TR nd = 0, na = 0, ng = 0;
-  if (p + 1 < P) {
-    const int sl = t + ps;
-    nd = q[sl]; na = q[T + sl]; ng = q[2 * T + sl];
+  const int P = L.P;
+  const bool wave = WAVE && wave_local(L);
+  TR nx[3];
+  {
+    const TR f[3] = {f_d, f_a, f_g};
+    from_above<3>(L, ring, wave, f, nx);
   }
-  ring.release();
+  const TR nd = nx[0], na = nx[1], ng = nx[2];
   TR A = al_e, Cc = -ga_e * ng, D = de_e - ga_e * nd;
   {
     const TR inv = TR(1) / (TR(1) - ga_e * na);
     A *= inv; Cc *= inv; D *= inv;
   }
   for (int h = 1; h < P; h <<= 1) {
-    TR *Q = ring.next<TR>();
-    Q[t] = A; Q[T + t] = Cc; Q[2 * T + t] = D;
-    __syncthreads();
-    TR Am = 0, Cm = 0, Dm = 0, Ap = 0, Cp = 0, Dp = 0;
-    if (p - h >= 0) {
-      const int sl = t - h * ps;
-      Am = Q[sl]; Cm = Q[T + sl]; Dm = Q[2 * T + sl];
-    }
-    if (p + h < P) {
-      const int sl = t + h * ps;
-      Ap = Q[sl]; Cp = Q[T + sl]; Dp = Q[2 * T + sl];
-    }
-    ring.release();
+    const TR v[3] = {A, Cc, D};
+    TR m[3], q[3];
+    from_both<3>(L, ring, wave, h, v, m, q);
+    const TR Am = m[0], Cm = m[1], Dm = m[2], Ap = q[0], Cp = q[1], Dp = q[2];
     const TR inv = TR(1) / (TR(1) - A * Cm - Cc * Ap);
     const TR nA = -inv * A * Am;
     const TR nC = -inv * Cc * Cp;
@@ -352,11 +446,9 @@ __device__ __forceinline__ void reduced_solve(const Lane &L, Ring &ring, TR al_e
     A = nA; Cc = nC; D = nD;
   }
   y = D;
-  TR *Q = ring.next<TR>();
-  Q[t] = y;
-  __syncthreads();
-  ym = (p > 0) ? Q[t - ps] : TR(0);
-  ring.release();
+  TR yv[1] = {y}, ymv[1];
+  from_below<1>(L, ring, wave, yv, ymv);
+  ym = ymv[0];
 }
 
 // (iv) x_r = dp_r - ap_r ym - gp_r y; x_{M-1} = y.
@@ -369,8 +461,8 @@ __device__ __forceinline__ void substitute(const TR (&ra)[M], const TR (&rc)[M],
 }
 
 // Whole solve of the lane's chunk, given a, b, c, d in chunk order; x in
-// chunk order on return. a, c, d are overwritten.
-template <typename TR, int M>
+// chunk order on return. a, c, d are overwritten. WAVE as for reduced_solve.
+template <typename TR, int M, bool WAVE = false>
 __device__ __forceinline__ void solve_chunk(const Lane &L, Ring &ring, TR (&ra)[M],
                                             const TR (&rb)[M], TR (&rc)[M], TR (&rd)[M],
                                             TR (&x)[M]) {
@@ -379,7 +471,7 @@ __device__ __forceinline__ void solve_chunk(const Lane &L, Ring &ring, TR (&ra)[
   TR f_d, f_a, f_g;
   backward_spike<TR, M>(ra, rc, rd, f_d, f_a, f_g);
   TR y, ym;
-  reduced_solve<TR>(L, ring, al_e, ga_e, de_e, f_d, f_a, f_g, y, ym);
+  reduced_solve<TR, WAVE>(L, ring, al_e, ga_e, de_e, f_d, f_a, f_g, y, ym);
   substitute<TR, M>(ra, rc, rd, y, ym, x);
 }
 

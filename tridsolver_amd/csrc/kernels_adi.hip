@@ -225,7 +225,7 @@ __global__ __launch_bounds__((AdiBounds<REAL, M>::threads)) void adi_solve_kerne
   hyb::to_chunk<M, CONTIG>(L, rd, tile);
   adi_coefs<REAL, M, CONTIG>(L, kc, ra, rb, rc);
   REAL x[M];
-  hyb::solve_chunk<REAL, M>(L, ring, ra, rb, rc, rd, x);
+  hyb::solve_chunk<REAL, M, CONTIG>(L, ring, ra, rb, rc, rd, x);
   hyb::to_fetch<M, CONTIG>(L, x, tile);
   if constexpr (RHS) {  // 32-bit byte offsets, as the loads
     if (!L.active) return;
@@ -359,6 +359,11 @@ tridStatus_t adi_sweep(int dev, REAL lambda, REAL *d, REAL *u, int nx, int ny, i
   HybridCfg cfg;
   if (!(sd == 0 && adi_x_cfg<REAL>(g, &cfg)) && !adi_cfg<REAL, INC>(dev, g, &cfg))
     return TRID_STATUS_INVALID_VALUE;
+  // The sweep forms coefficients from grid coordinates, so it is not split
+  // into sub-batches: its grid must fit HIP's 2^32 work-items (about 2^35
+  // points at 8 rows per lane, beyond a GPU's memory for two arrays).
+  if ((g.n0 + cfg.W - 1) / cfg.W * g.n1 > (int64_t)UINT32_MAX / (cfg.W * cfg.P))
+    return TRID_STATUS_INVALID_VALUE;
   AdiCoef<REAL> kc{lambda, nx, ny, nz, sd};
   hipStream_t st = current_stream();
   const bool contig = sd == 0;
@@ -472,6 +477,8 @@ tridStatus_t run(const TridParams *ctx, REAL lambda, REAL *u, REAL *du, REAL *a,
   } else {
     // Capture one step on a private stream (the null stream cannot be
     // captured), replay it `iters` times on the caller's stream.
+    int dev = 0;
+    if ((s = check_device(&dev)) != TRID_STATUS_SUCCESS) return s;
     hipStream_t cap = nullptr;
     if (hipStreamCreateWithFlags(&cap, hipStreamNonBlocking) != hipSuccess)
       return TRID_STATUS_EXECUTION_FAILED;
@@ -479,6 +486,24 @@ tridStatus_t run(const TridParams *ctx, REAL lambda, REAL *u, REAL *du, REAL *a,
     hipGraphExec_t exec = nullptr;
     void *prev = nullptr;
     tridHipGetStream(&prev);
+    if (a) {
+      // Solves that take the per-lane path (ctx->opts 1 or 2, or systems
+      // longer than one workgroup holds) need a workspace, which cannot be
+      // allocated during capture: reserve it for the capture stream now.
+      const int dims[3] = {nx, ny, nz};
+      size_t need = 0;
+      HybridCfg xc;
+      Geom gx;
+      bool empty = false;
+      const bool x_forced = forced_config().M == 0 &&
+                            make_geom(3, 0, dims, dims, &gx, &empty) == TRID_STATUS_SUCCESS &&
+                            !empty && adi_x_cfg<REAL>(gx, &xc);
+      if (!x_forced) need = lane_workspace_need<REAL, false>(dev, ctx, 3, 0, dims, dims);
+      need = std::max(need, lane_workspace_need<REAL, false>(dev, ctx, 3, 1, dims, dims));
+      need = std::max(need, lane_workspace_need<REAL, true>(dev, ctx, 3, 2, dims, dims));
+      void *ws = nullptr;
+      if (need > 0) s = get_workspace(dev, cap, need, &ws);
+    }
     if (s == TRID_STATUS_SUCCESS) {
       if (hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) != hipSuccess) {
         s = TRID_STATUS_EXECUTION_FAILED;
@@ -504,6 +529,7 @@ tridStatus_t run(const TridParams *ctx, REAL lambda, REAL *u, REAL *du, REAL *a,
     }
     if (graph) (void)hipGraphDestroy(graph);
     (void)hipStreamDestroy(cap);
+    drop_stream_workspaces(dev, cap);
   }
   if (s == TRID_STATUS_SUCCESS && sync && hipStreamSynchronize(st) != hipSuccess)
     s = TRID_STATUS_EXECUTION_FAILED;

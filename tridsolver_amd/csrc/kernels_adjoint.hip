@@ -55,7 +55,7 @@ __global__ __launch_bounds__((AdjBounds<REAL, M>::threads)) void adjoint_kernel(
   hyb::to_chunk<M, CONTIG>(L, rd, tile);
   if (GRAD) hyb::to_chunk<M, CONTIG>(L, rx, tile);
   REAL l[M];
-  hyb::solve_chunk<REAL, M>(L, ring, ra, rb, rc, rd, l);
+  hyb::solve_chunk<REAL, M, CONTIG>(L, ring, ra, rb, rc, rd, l);
   if (!GRAD) {
     hyb::to_fetch<M, CONTIG>(L, l, tile);
     hyb::store<M, CONTIG>(L, lam, l);

@@ -235,7 +235,7 @@ __global__ __launch_bounds__(T) void rg_solve(
     hyb::to_chunk<M, true>(L, rc, tile);
     hyb::to_chunk<M, true>(L, rd, tile);
     REAL x[M];
-    hyb::solve_chunk<REAL, M>(L, ring, ra, rb, rc, rd, x);
+    hyb::solve_chunk<REAL, M, true>(L, ring, ra, rb, rc, rd, x);
     hyb::to_fetch<M, true>(L, x, tile);
     if (INC) hyb::store_add<M, true>(L, u, x);
     else hyb::store<M, true>(L, d, x);

@@ -180,7 +180,7 @@ __global__ __launch_bounds__((HybridBounds<REAL, M>::threads)) void hybrid_kerne
     hyb::to_chunk<M, CONTIG>(L, rd, tile);
   }
   REAL x[M];
-  hyb::solve_chunk<REAL, M>(L, ring, ra, rb, rc, rd, x);
+  hyb::solve_chunk<REAL, M, CONTIG>(L, ring, ra, rb, rc, rd, x);
   if constexpr (VEC) {
     hyb::vto_fetch<M, V>(L, x, tile);
     hyb::vstore<M, V, INC>(L, INC ? u : d, x);

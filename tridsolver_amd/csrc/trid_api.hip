@@ -191,7 +191,10 @@ tridStatus_t solve(const TridParams *ctx, const REAL *a, const REAL *b, const RE
 
   if (use_hybrid) {
     record_config(3, &cfg);
-    return finish(ctx, st, launch_hybrid<REAL, INC>(g, cfg, a, b, c, d, u, st));
+    return finish(ctx, st, for_each_grid_chunk(g, cfg.W, cfg.W * cfg.P,
+                                               [&](const Geom &sub, int64_t o) {
+      return launch_hybrid<REAL, INC>(sub, cfg, a + o, b + o, c + o, d + o, advance(u, o), st);
+    }));
   }
   record_config(1, nullptr);
   const size_t one = sizeof(REAL) * (size_t)g.nsys() * (size_t)g.N;
@@ -202,6 +205,39 @@ tridStatus_t solve(const TridParams *ctx, const REAL *a, const REAL *b, const RE
   return finish(ctx, st, launch_lane<REAL, INC>(g, a, b, c, d, u, cws, dws, st));
 }
 
+// Workspace bytes the per-lane path would take for this call on this thread,
+// 0 when the call runs a hybrid kernel (the decision solve() makes). A
+// caller that captures such a call reserves this much for the capturing
+// stream first.
+template <typename REAL, bool INC>
+size_t lane_workspace_need(int dev, const TridParams *ctx, int ndim, int solvedim,
+                           const int *dims, const int *pads) {
+  Geom g;
+  bool empty = false;
+  if (make_geom(ndim, solvedim, dims, pads, &g, &empty) != TRID_STATUS_SUCCESS || empty)
+    return 0;
+  if (tl_forced.M != 0) return 0;
+  const int hint = ctx ? ctx->opts[solvedim] : 0;
+  HybridCfg cfg;
+  if (hint != 1 && hint != 2 && tuned_config<REAL, INC>(dev, g, &cfg)) return 0;
+  const size_t one = sizeof(REAL) * (size_t)g.nsys() * (size_t)g.N;
+  return INC ? 2 * one : one;
+}
+
+// Frees the workspaces of a stream that is being destroyed (after its work
+// has completed) and forgets them.
+void drop_stream_workspaces(int dev, hipStream_t st) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (auto it = g_ws.begin(); it != g_ws.end();) {
+    if (std::get<0>(it->first) == dev && std::get<1>(it->first) == st) {
+      (void)hipFree(it->second.ptr);
+      it = g_ws.erase(it);
+    } else {
+      ++it;
+    }
+  }
+}
+
 #define TRID_SOLVE_INST(REAL, INC)                                                          \
   template tridStatus_t solve<REAL, INC>(const TridParams *, const REAL *, const REAL *,   \
                                          const REAL *, REAL *, REAL *, int, int,           \
@@ -210,6 +246,14 @@ TRID_SOLVE_INST(float, false)
 TRID_SOLVE_INST(float, true)
 TRID_SOLVE_INST(double, false)
 TRID_SOLVE_INST(double, true)
+template size_t lane_workspace_need<float, false>(int, const TridParams *, int, int, const int *,
+                                                  const int *);
+template size_t lane_workspace_need<float, true>(int, const TridParams *, int, int, const int *,
+                                                 const int *);
+template size_t lane_workspace_need<double, false>(int, const TridParams *, int, int,
+                                                   const int *, const int *);
+template size_t lane_workspace_need<double, true>(int, const TridParams *, int, int, const int *,
+                                                  const int *);
 
 }  // namespace trid
 

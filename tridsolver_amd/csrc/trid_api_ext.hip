@@ -103,7 +103,11 @@ tridStatus_t adjoint(const TridParams *ctx, const REAL *a, const REAL *b, const 
     return TRID_STATUS_INVALID_VALUE;  // longer than one workgroup can hold
   }
   record_config(4, &cfg);
-  return finish(ctx, st, launch_adjoint<REAL>(g, cfg, a, b, c, x, w, lam, ga, gb, gc, st));
+  return finish(ctx, st, for_each_grid_chunk(g, cfg.W, cfg.W * cfg.P,
+                                             [&](const Geom &sub, int64_t o) {
+    return launch_adjoint<REAL>(sub, cfg, a + o, b + o, c + o, advance(x, o), w + o, lam + o,
+                                advance(ga, o), advance(gb, o), advance(gc, o), st);
+  }));
 }
 
 tridStatus_t ragged_mixed32(const TridParams *ctx, const float *a, const float *b,
@@ -161,7 +165,10 @@ EXTERN_C tridStatus_t tridDmtsvStridedBatchMixed(const TridParams *ctx, const do
     return TRID_STATUS_INVALID_VALUE;
   }
   record_config(2, &cfg);
-  return finish(ctx, st, launch_mixed(g, cfg, refinements, a, b, c, d, st));
+  return finish(ctx, st, for_each_grid_chunk(g, cfg.W, cfg.W * cfg.P,
+                                             [&](const Geom &sub, int64_t o) {
+    return launch_mixed(sub, cfg, refinements, a + o, b + o, c + o, d + o, st);
+  }));
 }
 
 EXTERN_C tridStatus_t tridDmtsvStridedBatchMixedF32(const TridParams *ctx, const float *a,
@@ -188,7 +195,10 @@ EXTERN_C tridStatus_t tridDmtsvStridedBatchMixedF32(const TridParams *ctx, const
     return TRID_STATUS_INVALID_VALUE;
   }
   record_config(10, &cfg);
-  return finish(ctx, st, launch_mixed32(g, cfg, refinements, a, b, c, d, x, st));
+  return finish(ctx, st, for_each_grid_chunk(g, cfg.W, cfg.W * cfg.P,
+                                             [&](const Geom &sub, int64_t o) {
+    return launch_mixed32(sub, cfg, refinements, a + o, b + o, c + o, d + o, x + o, st);
+  }));
 }
 
 EXTERN_C tridStatus_t tridDmtsvStridedBatchAdjoint(const TridParams *ctx, const double *a,

@@ -74,6 +74,12 @@ hipStream_t current_stream();
 tridStatus_t check_device(int *dev);
 // role 0: the solve's / ragged plan's workspace; 1: partitioned-solve saves
 tridStatus_t get_workspace(int dev, hipStream_t st, size_t bytes, void **out, int role = 0);
+// Role-0 workspace bytes the per-lane path would take for this call (0 when
+// it runs a hybrid kernel), and the release of a destroyed stream's buffers.
+template <typename REAL, bool INC>
+size_t lane_workspace_need(int dev, const TridParams *ctx, int ndim, int solvedim,
+                           const int *dims, const int *pads);
+void drop_stream_workspaces(int dev, hipStream_t st);
 
 // ---- streamed strided variant (kernels_stream.hip) ------------------------
 template <typename REAL>
@@ -150,6 +156,47 @@ inline hipError_t ensure_lds(const void *kern, size_t lds, LdsGrant &granted) {
   e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e == hipSuccess) granted.bytes[dev].store(lds);
   return e;
+}
+
+// A HIP grid holds fewer than 2^32 work-items (gridDim.x * blockDim.x). The
+// batched kernels launch ceil(n0 / W) * n1 workgroups of T threads, which can
+// pass that for 2^31 short systems. This calls fn(sub, off) once per
+// sub-batch whose grid fits, where `sub` covers a range of i1 (or, when one
+// i1 slice alone is too large, a range of i0 in whole workgroups) and `off`
+// is the element offset of its first system: the caller launches on `sub`
+// with every array pointer advanced by `off`. One call when the grid fits.
+template <class F>
+hipError_t for_each_grid_chunk(const Geom &g, int W, int T, F &&fn) {
+  const int64_t max_blocks = (int64_t)UINT32_MAX / T;
+  const int64_t nb0 = (g.n0 + W - 1) / W;
+  if (nb0 * g.n1 <= max_blocks) return fn(g, int64_t(0));
+  if (nb0 <= max_blocks) {
+    const int64_t per = max_blocks / nb0;
+    for (int64_t i1 = 0; i1 < g.n1; i1 += per) {
+      Geom s = g;
+      s.n1 = g.n1 - i1 < per ? g.n1 - i1 : per;
+      const hipError_t e = fn(s, i1 * g.st1);
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+  }
+  const int64_t per0 = max_blocks * W;
+  for (int64_t i1 = 0; i1 < g.n1; ++i1) {
+    for (int64_t i0 = 0; i0 < g.n0; i0 += per0) {
+      Geom s = g;
+      s.n0 = g.n0 - i0 < per0 ? g.n0 - i0 : per0;
+      s.n1 = 1;
+      const hipError_t e = fn(s, i0 * g.st0 + i1 * g.st1);
+      if (e != hipSuccess) return e;
+    }
+  }
+  return hipSuccess;
+}
+
+// p + off, or nullptr for an absent (null) array.
+template <typename T>
+inline T *advance(T *p, int64_t off) {
+  return p ? p + off : nullptr;
 }
 
 // ---- shared entry-point helpers (trid_api.hip) ----------------------------
