@@ -206,6 +206,23 @@ EXTERN_C tridStatus_t tridDmtsvRaggedBatchMixedF32(const TridParams *ctx, const 
                                                   const int64_t *offsets, int64_t nsys,
                                                   int64_t max_len, int refinements);
 EXTERN_C size_t tridHipRaggedWorkspaceBytes(int64_t nsys);
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): tridDmtsvRaggedBatch with its plan in a fixed array the
+ * library holds per GPU (file-scope device memory of about 4 MiB, kept until
+ * the process ends) instead of the stream's workspace. It calls no hipMalloc,
+ * hipFree or allocating routine and never runs the autotuner (it takes a
+ * cached shape or the default). nsys may be at most
+ * TRID_HIP_NOALLOC_MAX_SYSTEMS; larger batches return
+ * TRID_STATUS_INVALID_VALUE before anything is launched. Every call on one
+ * GPU plans in that GPU's single array: calls that overlap on one GPU (other
+ * streams or host threads) race and must be serialised by the caller.
+ * Results equal tridDmtsvRaggedBatch's. */
+#define TRID_HIP_NOALLOC_MAX_SYSTEMS 1048576
+EXTERN_C tridStatus_t tridDmtsvRaggedBatchNoAlloc(const TridParams *ctx, const double *a,
+                                                  const double *b, const double *c, double *d,
+                                                  const int64_t *offsets, int64_t nsys,
+                                                  int64_t max_len);
 /* Forces the ragged kernel's rows per lane (4, 8 or 16), workgroups per CU
  * and solve launches (1; or 2: systems needing more than half a workgroup's
  * lanes, then the rest in half-size workgroups, two per CU) for this thread's

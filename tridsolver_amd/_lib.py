@@ -81,6 +81,7 @@ SIGNATURES = {
     "tridSmtsvStridedBatchIncPartSolve": ([P] * 6 + [I, I, P, P, I, I, P], I),
     "tridSmtsvRaggedBatchInc": ([P] * 7 + [L, L], I),
     "tridHipRaggedWorkspaceBytes": ([L], SZ),
+    "tridDmtsvRaggedBatchNoAlloc": ([P] * 6 + [L, L], I),
     "tridHipSetAutotuneMode": ([I], I),
     "tridHipAutotune": ([I, I, I, I, P, P, C.POINTER(C.c_float)], I),
     "tridHipAutotuneMixed": ([I, I, P, P, C.POINTER(C.c_float)], I),

@@ -248,7 +248,7 @@ __global__ __launch_bounds__(T) void rg_solve(
 void launch_plan(const int64_t *off, int64_t S, int M, int Tmax, int Tshort, void *ws,
                  hipStream_t st) {
   RaggedPlan *plan = static_cast<RaggedPlan *>(ws);
-  int *list = reinterpret_cast<int *>(static_cast<unsigned char *>(ws) + 512);
+  int *list = reinterpret_cast<int *>(static_cast<unsigned char *>(ws) + RAGGED_PLAN_HEADER_BYTES);
   hipLaunchKernelGGL(rg_reset, dim3(1), dim3(64), 0, st, plan);
   const unsigned nb = (unsigned)((S + RG_PLAN_BLOCK - 1) / RG_PLAN_BLOCK);
   if (nb > 0) {
@@ -269,7 +269,7 @@ hipError_t launch_m(const REAL *a, const REAL *b, const REAL *c, REAL *d, REAL *
   constexpr int Tmax = RgBounds<REAL, M>::threads;
   constexpr int Tshort = Tmax / 2;
   RaggedPlan *plan = static_cast<RaggedPlan *>(ws);
-  int *list = reinterpret_cast<int *>(static_cast<unsigned char *>(ws) + 512);
+  int *list = reinterpret_cast<int *>(static_cast<unsigned char *>(ws) + RAGGED_PLAN_HEADER_BYTES);
   int cus = 256;
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   const int per = std::max(1, per_cu);
@@ -402,7 +402,7 @@ hipError_t launch_mixed32_one(const float *a, const float *b, const float *c, co
                               void *ws, hipStream_t st, int dev) {
   constexpr int Tmax = RG_MIXED_THREADS;
   RaggedPlan *plan = static_cast<RaggedPlan *>(ws);
-  int *list = reinterpret_cast<int *>(static_cast<unsigned char *>(ws) + 512);
+  int *list = reinterpret_cast<int *>(static_cast<unsigned char *>(ws) + RAGGED_PLAN_HEADER_BYTES);
   launch_plan(off, S, RG_M32_M, Tmax, Tmax, ws, st);  // one launch: no long range
   const size_t lds =
       sizeof(double) * (size_t)Tmax * (RG_M32_M + 1) + hyb::ring_bytes<double>(Tmax, 2);
@@ -425,7 +425,7 @@ hipError_t launch_mixed_m(const double *a, const double *b, const double *c, dou
                           hipStream_t st, int dev) {
   constexpr int Tmax = RG_MIXED_THREADS;
   RaggedPlan *plan = static_cast<RaggedPlan *>(ws);
-  int *list = reinterpret_cast<int *>(static_cast<unsigned char *>(ws) + 512);
+  int *list = reinterpret_cast<int *>(static_cast<unsigned char *>(ws) + RAGGED_PLAN_HEADER_BYTES);
   launch_plan(off, S, M, Tmax, Tmax, ws, st);  // one launch: no long range
   const size_t lds =
       sizeof(double) * (size_t)Tmax * (M + 1) + hyb::ring_bytes<double>(Tmax, 3);
@@ -515,7 +515,7 @@ hipError_t launch_mixed_lds(const double *a, const double *b, const double *c, d
                             void *ws, hipStream_t st, int dev) {
   constexpr int Tmax = 512, Tshort = 256;
   RaggedPlan *plan = static_cast<RaggedPlan *>(ws);
-  int *list = reinterpret_cast<int *>(static_cast<unsigned char *>(ws) + 512);
+  int *list = reinterpret_cast<int *>(static_cast<unsigned char *>(ws) + RAGGED_PLAN_HEADER_BYTES);
   int cus = 256;
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   const int per = std::max(1, per_cu);
@@ -562,7 +562,9 @@ hipError_t launch_ragged_mixed(int M, const double *a, const double *b, const do
   }
 }
 
-size_t ragged_workspace_bytes(int64_t S) { return 512 + sizeof(int) * (size_t)std::max<int64_t>(S, 1); }
+size_t ragged_workspace_bytes(int64_t S) {
+  return RAGGED_PLAN_HEADER_BYTES + sizeof(int) * (size_t)std::max<int64_t>(S, 1);
+}
 
 int ragged_max_len(int fp64, int M) {
   (void)fp64;  // the same shapes at both precisions
@@ -574,7 +576,7 @@ template <typename REAL, bool INC>
 hipError_t launch_ragged(int M, const REAL *a, const REAL *b, const REAL *c, REAL *d, REAL *u,
                          const int64_t *off, int64_t S, int per_cu, int launches, void *ws,
                          hipStream_t st, int dev) {
-  static_assert(sizeof(RaggedPlan) <= 512, "plan header");
+  static_assert(sizeof(RaggedPlan) <= RAGGED_PLAN_HEADER_BYTES, "plan header");
   if (launches != 1 && launches != 2) return hipErrorInvalidValue;
   switch (M) {
     case 4: return launch_m<REAL, 4, INC>(a, b, c, d, u, off, S, per_cu, launches, ws, st, dev);

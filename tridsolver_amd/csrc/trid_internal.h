@@ -119,6 +119,9 @@ hipError_t launch_adjoint(const Geom &g, const HybridCfg &cfg, const REAL *a, co
                           REAL *gb, REAL *gc, hipStream_t st);
 
 // ---- ragged batch (kernels_ragged.hip) ------------------------------------
+// A ragged batch plans in ragged_workspace_bytes(S) bytes: this header (plan
+// counters and tables) followed by one 32-bit slot per system.
+constexpr size_t RAGGED_PLAN_HEADER_BYTES = 512;
 int ragged_mixed_max_len(int M);
 int ragged_mixed32_max_len();
 hipError_t launch_ragged_mixed32(const float *a, const float *b, const float *c, const float *d,
