@@ -52,8 +52,11 @@ def parse():
     ap.add_argument("--cpu-slab", type=int, default=64,
                     help="z-extent of the CPU baseline sample (n x n x slab)")
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--access-dirs", default="yz", choices=["y", "z", "yz"],
+                    help="directions of --only access")
     ap.add_argument("--only", choices=["fp32", "mixed", "mixed-f32", "adjoint", "ragged", "adi",
-                                       "partitioned", "cpu-extras", "oracle-checks"],
+                                       "partitioned", "cpu-extras", "oracle-checks", "access",
+                                       "copy"],
                     help="run one secondary workload and print its JSON instead of the bench "
                          "line (for profiling)")
     return ap.parse_args()
@@ -89,15 +92,69 @@ def event_ms(torch, fn, reps=5, warm=1):
     return statistics.median(ts)
 
 
+COPY_SHAPES = [(t, v, nt) for t in (256, 512, 1024) for v in (1, 2, 4) for nt in (1, 0)]
+
+
 def measure_copy_bw(torch, lib, nbytes=4 << 30, reps=10):
-    """HBM bandwidth of the library's stream-copy kernel (read + write bytes)."""
+    """HBM bandwidth of the library's stream-copy kernel (read + write bytes):
+    the best over a sweep of shapes (threads per workgroup, 16-byte vectors
+    per thread, non-temporal or plain accesses). Returns (GB/s, shape, all)."""
     src = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").fill_(1.0)
     dst = torch.empty_like(src)
-    ms = event_ms(torch, lambda: lib.tridHipStreamCopy(C.c_void_p(src.data_ptr()),
-                                                       C.c_void_p(dst.data_ptr()), nbytes),
-                  reps=reps, warm=2)
+    res = {}
+    for t, v, nt in COPY_SHAPES:
+        ms = event_ms(torch, lambda: lib.tridHipStreamCopyShape(
+            C.c_void_p(src.data_ptr()), C.c_void_p(dst.data_ptr()), nbytes, t, v, nt),
+            reps=reps, warm=2)
+        res["%d/%d/%s" % (t, v, "nt" if nt else "plain")] = round(2 * nbytes / (ms * 1e-3) / 1e9, 1)
     del src, dst
-    return 2 * nbytes / (ms * 1e-3) / 1e9
+    best = max(res, key=res.get)
+    return res[best], best, res
+
+
+def bench_access(torch, api, tune, dirs="yz", n=512):
+    """Bandwidth ceiling of the strided solve's access pattern
+    (tridHipAccessProbe: the solve's loads and stores, an add per value in
+    place of its arithmetic) along y and z of an n^3 FP64 grid, for 128- and
+    256-byte row segments, 16 and 32 rows per lane and XCD groupings 1, 8,
+    16; 40 B per point as the solve. Beside it the solve of the same
+    direction (tuned for the geometry first with tune, else the default
+    shape). dirs: the directions to measure ("y", "z" or "yz")."""
+    lib = api._lib.lib()
+    n3 = n ** 3
+    dims = [n, n, n]
+    a, b, c, d = gen_gpu(torch, n3, torch.float64, seed=512)
+    out = {}
+    for sd, name in ((1, "y"), (2, "z")):
+        if name not in dirs:
+            continue
+        rows = {}
+        for seg in (128, 256):
+            for M in (16, 32):
+                for G in (1, 8, 16):
+                    st = lib.tridHipAccessProbe(api._ptr(a), api._ptr(b), api._ptr(c),
+                                                api._ptr(d), sd, api.int3(dims), seg, M, G)
+                    if st != 0:
+                        continue
+                    ms = event_ms(torch, lambda: lib.tridHipAccessProbe(
+                        api._ptr(a), api._ptr(b), api._ptr(c), api._ptr(d), sd,
+                        api.int3(dims), seg, M, G), reps=9)
+                    rows["seg%d_M%d_G%d" % (seg, M, G)] = {
+                        "ms": round(ms, 4), "alg_GBs": round(40.0 * n3 / ms / 1e6, 1)}
+        d.copy_(a)
+        if tune:
+            api.autotune(torch.float64, sd, dims)
+        ms = event_ms(torch, lambda: api.trid_strided_batch(a, b, c, d, solvedim=sd, dims=dims),
+                      reps=9)
+        best = max(rows, key=lambda k: rows[k]["alg_GBs"])
+        out[name] = {"probe": rows, "probe_best": best,
+                     "probe_best_GBs": rows[best]["alg_GBs"],
+                     "solve_ms": round(ms, 4), "solve_alg_GBs": round(40.0 * n3 / ms / 1e6, 1),
+                     "solve_config": api.last_config(),
+                     "solve_over_probe_best": round(40.0 * n3 / ms / 1e6 / rows[best]["alg_GBs"], 3)}
+    del a, b, c, d
+    torch.cuda.empty_cache()
+    return out
 
 
 def cpu_baseline(n, slab, reps=5):
@@ -414,13 +471,20 @@ def bench_mixed_f32(torch, api, n, fp64_ms, tune=False):
     return out
 
 
-def bench_adjoint(torch, api, n=256):
+def bench_adjoint(torch, api, tune, n=256):
+    """Adjoint / forward time at n^3 FP64 per direction. With tune, the
+    forward solve and both adjoint variants are autotuned for the geometry
+    first, so the ratio compares tuned shapes."""
     n3 = n ** 3
     dims = [n, n, n]
     a, b, c, d = gen_gpu(torch, n3, torch.float64, seed=256)
     w = torch.randn(n3, dtype=torch.float64, device="cuda")
     out = {}
     for sd, name in enumerate("xyz"):
+        if tune:
+            api.autotune(torch.float64, sd, dims)
+            api.autotune_adjoint(torch.float64, sd, dims, grads=True)
+            api.autotune_adjoint(torch.float64, sd, dims, grads=False)
         x = d.clone()
         fwd = event_ms(torch, lambda: api.trid_strided_batch(a, b, c, x, solvedim=sd, dims=dims))
         x = d.clone()
@@ -431,7 +495,9 @@ def bench_adjoint(torch, api, n=256):
         adj_d = event_ms(torch, lambda: api.adjoint_strided_batch(
             a, b, c, x, w, solvedim=sd, dims=dims, lam=lam))
         out[name] = {"fwd_ms": round(fwd, 4), "adj_all_ms": round(adj_all, 4),
-                     "adj_dL_dd_ms": round(adj_d, 4),
+                     "adj_dL_dd_ms": round(adj_d, 4), "tuned": bool(tune),
+                     "adj_all_alg_GBs": round(72.0 * n3 / adj_all / 1e6, 1),
+                     "fwd_alg_GBs": round(40.0 * n3 / fwd / 1e6, 1),
                      "adj_over_fwd_all_grads": round(adj_all / fwd, 3),
                      "adj_over_fwd_dL_dd_only": round(adj_d / fwd, 3),
                      "byte_floor_all_grads": 72 / 40, "byte_floor_dL_dd_only": 40 / 40}
@@ -626,12 +692,14 @@ def main():
                 bench_mixed_f32(torch, api, n, ref, tune)
         else:
             run = {"fp32": lambda: bench_fp32_dirs(torch, api, n, tune),
-                   "adjoint": lambda: bench_adjoint(torch, api),
+                   "adjoint": lambda: bench_adjoint(torch, api, tune),
                    "ragged": lambda: bench_ragged(torch, api, tune),
                    "adi": lambda: bench_adi(torch),
                    "partitioned": lambda: bench_partitioned(torch, api),
                    "cpu-extras": cpu_extras,
-                   "oracle-checks": lambda: oracle_checks(torch, api, n)}[args.only]
+                   "oracle-checks": lambda: oracle_checks(torch, api, n),
+                   "access": lambda: bench_access(torch, api, tune, args.access_dirs, n),
+                   "copy": lambda: measure_copy_bw(torch, lib)}[args.only]
             res = run()
         print(json.dumps({"only": args.only, "result": res}))
         return
@@ -641,8 +709,8 @@ def main():
     tune = not args.no_tune
 
     api.use_torch_stream()  # library launches on torch's current stream
-    copy_bw = measure_copy_bw(torch, lib)
-    log("copy bandwidth %.1f GB/s" % copy_bw)
+    copy_bw, copy_shape, copy_sweep = measure_copy_bw(torch, lib)
+    log("copy bandwidth %.1f GB/s (threads/vectors/access %s)" % (copy_bw, copy_shape))
 
     if tune:
         for sd in range(3):
@@ -704,7 +772,7 @@ def main():
         log("mixed: %s" % extras["fp32_refined_vs_fp64"])
         extras["fp32_stored_refined_vs_fp64"] = bench_mixed_f32(torch, api, n, per_dir_ms, tune)
         log("mixed, FP32-stored: %s" % extras["fp32_stored_refined_vs_fp64"])
-        extras["adjoint_256"] = bench_adjoint(torch, api)
+        extras["adjoint_256"] = bench_adjoint(torch, api, not args.no_tune)
         log("adjoint: %s" % extras["adjoint_256"])
         extras["ragged_40k"] = bench_ragged(torch, api, tune)
         log("ragged: %s" % extras["ragged_40k"])
@@ -757,6 +825,8 @@ def main():
                                 "per_dir_gpts_s", "nproc", "single_thread") if k in cpu},
         "per_direction": dirs,
         "copy_GBs": round(copy_bw, 1),
+        "copy_shape": copy_shape,
+        "copy_sweep_GBs": copy_sweep,
         "extras": extras,
     }
     print(json.dumps(out))

@@ -90,6 +90,10 @@ EXTERN_C tridStatus_t tridHipAutotuneMixed(int ndim, int solvedim, const int *di
 /* Tunes tridDmtsvStridedBatchMixedF32 (1 refinement) for one geometry. */
 EXTERN_C tridStatus_t tridHipAutotuneMixedF32(int ndim, int solvedim, const int *dims,
                                               const int *pads, float *best_ms);
+/* Tunes trid{D,S}mtsvStridedBatchAdjoint for one geometry (fp64: 1 double /
+ * 0 float; grads: 1 with the coefficient gradients, 0 for lambda only). */
+EXTERN_C tridStatus_t tridHipAutotuneAdjoint(int fp64, int grads, int ndim, int solvedim,
+                                             const int *dims, const int *pads, float *best_ms);
 /* Tunes the ragged kernel (rows per lane, workgroups per CU) on the caller's
  * device offsets with scratch values; cached per (precision, max_len). */
 EXTERN_C tridStatus_t tridHipAutotuneRagged(int fp64, const int64_t *offsets, int64_t nsys,
@@ -236,6 +240,23 @@ EXTERN_C tridStatus_t tridHipForceRaggedConfig(int M, int per_cu, int launches);
  * asynchronously. Used to measure the HBM bandwidth the solver's roofline is
  * quoted against and to calibrate hardware byte counters. */
 EXTERN_C tridStatus_t tridHipStreamCopy(const void *src, void *dst, size_t bytes);
+/* The same copy in a chosen shape: `threads` per workgroup (64..1024, whole
+ * waves), `vectors` 16-byte vectors per thread (1, 2 or 4, all loaded before
+ * the stores), non-temporal (1) or plain (0) accesses. A sweep of these gives
+ * the best copy bandwidth of the box. */
+EXTERN_C tridStatus_t tridHipStreamCopyShape(const void *src, void *dst, size_t bytes,
+                                             int threads, int vectors, int nontemporal);
+/* The loads and stores of the strided FP64 solve with seg_bytes-wide row
+ * segments (128 or 256: 16 or 32 x-neighbouring systems per workgroup) and M
+ * rows per lane (16 or 32) along solvedim 1 or 2 of a dense dims[0..2] grid:
+ * `a` by global->LDS loads, b, c, d into registers, d = a + b + c + d written
+ * back; G as the hybrid's XCD grouping. Reads 32 B and writes 8 B per point,
+ * like the solve, with none of its arithmetic: the bandwidth that access
+ * pattern allows. Grids below 4 GiB per array; workgroups of at most 1024
+ * threads (512 at M = 32). Asynchronous on this thread's stream. */
+EXTERN_C tridStatus_t tridHipAccessProbe(const double *a, const double *b, const double *c,
+                                         double *d, int solvedim, const int *dims, int seg_bytes,
+                                         int M, int G);
 
 /* ---- systems split across shards ------------------------------------------
  * No counterpart in the reference's single-GPU API; the role of its

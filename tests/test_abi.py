@@ -114,3 +114,24 @@ def test_force_hybrid_config_validation():
     assert lib.tridHipForceHybridConfig(8, 1, 1, 0, 0) == 3
     assert lib.tridHipForceHybridConfig(8, 1, 1, 1, 4) == 3
     assert lib.tridHipForceHybridConfig(0, 0, 0, 0, 0) == 0
+
+
+def test_measurement_entry_points_reject_bad_arguments():
+    lib = _lib.lib()
+    p = C.c_void_p(16)
+    dims = _lib.int3([64, 64, 64])
+    # copy shape: threads not whole waves / too many, vectors not 1, 2 or 4, size not 16-byte
+    assert lib.tridHipStreamCopyShape(p, p, 1 << 20, 100, 1, 1) == 3
+    assert lib.tridHipStreamCopyShape(p, p, 1 << 20, 2048, 1, 1) == 3
+    assert lib.tridHipStreamCopyShape(p, p, 1 << 20, 256, 3, 1) == 3
+    assert lib.tridHipStreamCopyShape(p, p, 1 << 20, 256, 8, 1) == 3
+    assert lib.tridHipStreamCopyShape(p, p, 1000, 256, 1, 1) == 3
+    # access probe: strided directions only, 128- or 256-byte rows, M 16 or 32
+    assert lib.tridHipAccessProbe(p, p, p, p, 0, dims, 256, 32, 1) == 3
+    assert lib.tridHipAccessProbe(p, p, p, p, 2, dims, 512, 32, 1) == 3
+    assert lib.tridHipAccessProbe(p, p, p, p, 2, dims, 256, 8, 1) == 3
+    assert lib.tridHipAccessProbe(p, p, p, p, 2, dims, 256, 32, 0) == 3
+    assert lib.tridHipAccessProbe(None, p, p, p, 2, dims, 256, 32, 1) == 3
+    # adjoint tuner: the geometry checks of the solve
+    assert lib.tridHipAutotuneAdjoint(1, 1, 3, 3, dims, dims, None) == 3
+    assert lib.tridHipAutotuneAdjoint(1, 1, 3, 0, _lib.int3([4, -1, 4]), dims, None) == 3

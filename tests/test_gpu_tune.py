@@ -160,3 +160,30 @@ def test_autotune_mixed_and_ragged():
     want = d.copy()
     oracle.ragged(a, b, c, want, off)
     H.assert_norm_close(td.cpu().numpy(), want, np.float64)
+
+
+@pytest.mark.parametrize("sd", [0, 1, 2])
+def test_autotune_adjoint(sd):
+    """The adjoint's tuned shapes (with and without the coefficient
+    gradients) are used by later calls and keep the results of the CPU
+    adjoint (normwise <= 1e-12 FP64)."""
+    dims = [72, 40, 48]
+    rng = np.random.default_rng(70 + sd)
+    n = int(np.prod(dims))
+    a, b, c, d = H.diag_dominant(rng, n)
+    w = rng.standard_normal(n)
+    x = d.copy()
+    oracle.batch(a, b, c, x, ndim=3, solvedim=sd, dims=dims)
+    ref = oracle.adjoint(a, b, c, x, w, ndim=3, solvedim=sd, dims=dims)
+    _lib.lib().tridHipAutotuneClear()
+    for grads in (True, False):
+        assert api.autotune_adjoint(torch.float64, sd, dims, grads=grads) > 0
+        want = True if grads else None
+        out = api.adjoint_strided_batch(dev(a), dev(b), dev(c), dev(x), dev(w), solvedim=sd,
+                                        dims=dims, grad_a=want, grad_b=want, grad_c=want,
+                                        sync=True)
+        assert api.last_config()[0] == 4
+        got = [t.cpu().numpy() for t in out if t is not None]
+        assert len(got) == (4 if grads else 1)
+        for g, r in zip(got, ref):
+            H.assert_norm_close(g, r, tol=1e-12)

@@ -64,6 +64,8 @@ SIGNATURES = {
     "tridHipLastConfig": ([IP, IP, IP, IP, IP], I),
     "tridHipForceHybridConfig": ([I, I, I, I, I], I),
     "tridHipStreamCopy": ([P, P, SZ], I),
+    "tridHipStreamCopyShape": ([P, P, SZ, I, I, I], I),
+    "tridHipAccessProbe": ([P, P, P, P, I, P, I, I, I], I),
     "tridDmtsvStridedBatchMixed": ([P, P, P, P, P, I, I, P, P, I], I),
     "tridDmtsvStridedBatchMixedF32": ([P, P, P, P, P, P, I, I, P, P, I], I),
     "tridDmtsvStridedBatchAdjoint": ([P] * 10 + [I, I, P, P], I),
@@ -87,6 +89,7 @@ SIGNATURES = {
     "tridHipAutotuneMixed": ([I, I, P, P, C.POINTER(C.c_float)], I),
     "tridHipAutotuneMixedF32": ([I, I, P, P, C.POINTER(C.c_float)], I),
     "tridHipAutotuneRagged": ([I, P, L, L], I),
+    "tridHipAutotuneAdjoint": ([I, I, I, I, P, P, C.POINTER(C.c_float)], I),
     "tridHipAutotuneSave": ([C.c_char_p], I),
     "tridHipAutotuneLoad": ([C.c_char_p], I),
     "tridHipAutotuneClear": ([], I),

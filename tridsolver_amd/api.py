@@ -360,6 +360,17 @@ def autotune_mixed_f32(solvedim, dims, pads=None):
     return ms.value
 
 
+def autotune_adjoint(dtype, solvedim, dims, pads=None, grads=True):
+    """Tunes the adjoint solve (with or without the coefficient gradients)
+    for one geometry; returns the best time in ms."""
+    pads = dims if pads is None else pads
+    ms = C.c_float(0)
+    check(_lib.lib().tridHipAutotuneAdjoint(1 if dtype == torch.float64 else 0,
+                                            1 if grads else 0, len(dims), solvedim, int3(dims),
+                                            int3(pads), C.byref(ms)), "tridHipAutotuneAdjoint")
+    return ms.value
+
+
 def autotune_ragged(dtype, offsets, max_len):
     check(_lib.lib().tridHipAutotuneRagged(1 if dtype == torch.float64 else 0, _ptr(offsets),
                                            offsets.numel() - 1, int(max_len)),

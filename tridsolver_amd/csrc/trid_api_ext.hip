@@ -99,7 +99,7 @@ tridStatus_t adjoint(const TridParams *ctx, const REAL *a, const REAL *b, const 
   if (forced.M != 0) {
     cfg = forced;
     if (!adjoint_cfg_valid<REAL>(g, cfg)) return TRID_STATUS_INVALID_VALUE;
-  } else if (!adjoint_default_cfg<REAL>(g, &cfg)) {
+  } else if (!adjoint_tuned_config<REAL>(dev, g, ga || gb || gc, &cfg)) {
     return TRID_STATUS_INVALID_VALUE;  // longer than one workgroup can hold
   }
   record_config(4, &cfg);

@@ -16,6 +16,11 @@ bool mixed_tuned_config(int dev, const Geom &g, HybridCfg *cfg);
 // Shape of the FP32-stored solve for g at `refine` refinements.
 bool mixed32_tuned_config(int dev, const Geom &g, int refine, HybridCfg *cfg);
 
+// Shape of the adjoint solve for g, with (grads) or without the coefficient
+// gradients; false if the systems are longer than one workgroup holds.
+template <typename REAL>
+bool adjoint_tuned_config(int dev, const Geom &g, bool grads, HybridCfg *cfg);
+
 // Rows per lane M, workgroups per CU and solve launches (1, or 2 with the
 // short systems apart) of the ragged persistent kernel for systems up to
 // max_len long.

@@ -15,8 +15,9 @@
 //   W systems per workgroup, G XCD grouping in {1, 8, 16},
 // skipping strided shapes whose rows are narrower than 64 bytes, plus, for
 // strided systems, the streamed variants and the variant with `a` staged in
-// LDS (128- and 256-byte rows, M in {16, 32}). The fastest is cached (the
-// fastest four of a first pass are timed again, median of 9).
+// LDS (128- and 256-byte rows, M in {16, 32}, G in {1, 2, 4, 8, 16, 32}). The
+// fastest is cached (the fastest four of a first pass are timed again, median
+// of 9). The adjoint is tuned the same way over its own kernel's shapes.
 // The cache can be saved to and loaded from a text file.
 
 #include <algorithm>
@@ -35,7 +36,7 @@
 namespace trid {
 namespace {
 
-enum Kind : int { KIND_SOLVE = 0, KIND_MIXED = 1, KIND_RAGGED = 2, KIND_MIXED32 = 3 };
+enum Kind : int { KIND_SOLVE = 0, KIND_MIXED = 1, KIND_RAGGED = 2, KIND_MIXED32 = 3, KIND_ADJOINT = 4 };
 
 struct Key {
   int dev, kind, fp64, inc;
@@ -48,7 +49,7 @@ struct Key {
 
 struct Choice {
   int algo;  // 1 lane, 3 hybrid (solve); 2 mixed; 5 ragged (cfg.M, cfg.G = per_cu,
-             // cfg.V = solve launches); 10 FP32-stored mixed
+             // cfg.V = solve launches); 10 FP32-stored mixed; 4 adjoint
   HybridCfg cfg;
   float ms;
 };
@@ -176,10 +177,10 @@ std::vector<HybridCfg> candidates(const Geom &g) {
           }
         }
       }
-  if (g.stride != 1) {  // wide variant: 128- or 256-byte rows, a in LDS
+  if (g.stride != 1) {  // wide variant: 128- or 256-byte rows, a in LDS; finer XCD groupings
     for (int seg : {128, 256})
       for (int M : {16, 32})
-        for (int G : Gs) {
+        for (int G : {1, 2, 4, 8, 16, 32}) {
           HybridCfg c;
           c.M = M;
           c.W = seg / (int)sizeof(REAL);
@@ -458,6 +459,89 @@ bool mixed32_tuned_config(int dev, const Geom &g, int refine, HybridCfg *cfg) {
   return mixed32_default_cfg(g, refine, cfg);
 }
 
+// Times every valid adjoint shape (M rows per lane, W systems per
+// workgroup, XCD grouping G) on scratch data of g's extent, with (grads) or
+// without the coefficient gradients; the fastest four again, median of 9.
+template <typename REAL>
+tridStatus_t tune_adjoint(int dev, const Geom &g, bool grads, Choice *best) {
+  (void)dev;
+  hipStream_t st = nullptr;
+  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess)
+    return TRID_STATUS_EXECUTION_FAILED;
+  const size_t n = extent_of(g);
+  const int arrays = grads ? 9 : 6;  // a b c x w lam (ga gb gc)
+  Scratch s;
+  if (hipMalloc(&s.mem, sizeof(REAL) * n * arrays) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipStreamDestroy(st);
+    return TRID_STATUS_ALLOC_FAILED;
+  }
+  REAL *a = static_cast<REAL *>(s.mem), *b = a + n, *c = b + n, *x = c + n, *w = x + n;
+  REAL *lam = w + n;
+  REAL *ga = grads ? lam + n : nullptr, *gb = grads ? lam + 2 * n : nullptr;
+  REAL *gc = grads ? lam + 3 * n : nullptr;
+  hipLaunchKernelGGL(tune_fill<REAL>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a,
+                     b, c, x, w, n);
+  best->algo = 0;
+  best->ms = INFINITY;
+  static const int Ms[] = {1, 2, 4, 8, 16};
+  static const int Ws[] = {1, 2, 4, 8, 16, 32, 64};
+  static const int Gs[] = {1, 8, 16};
+  std::vector<std::pair<float, HybridCfg>> timed;
+  for (int M : Ms)
+    for (int W : Ws)
+      for (int G : Gs) {
+        HybridCfg cfg;
+        cfg.M = M;
+        cfg.P = (int)((g.N + M - 1) / M);
+        cfg.W = W;
+        cfg.G = G;
+        if (g.stride != 1 && (int64_t)W * (int64_t)sizeof(REAL) < 64 && W < g.n0) continue;
+        if ((int64_t)W > g.n0 && W > 1) continue;
+        if ((int64_t)cfg.W * cfg.P < 32 && cfg.W < g.n0) continue;
+        if (!adjoint_cfg_valid<REAL>(g, cfg)) continue;
+        const float ms = time_on(st, [&] {
+          return launch_adjoint<REAL>(g, cfg, a, b, c, x, w, lam, ga, gb, gc, st);
+        });
+        if (std::isfinite(ms)) timed.emplace_back(ms, cfg);
+      }
+  std::sort(timed.begin(), timed.end(),
+            [](const std::pair<float, HybridCfg> &p, const std::pair<float, HybridCfg> &q) {
+              return p.first < q.first;
+            });
+  for (size_t k = 0; k < timed.size() && k < 4; ++k) {
+    const HybridCfg &cfg = timed[k].second;
+    const float ms = time_on(st, [&] {
+      return launch_adjoint<REAL>(g, cfg, a, b, c, x, w, lam, ga, gb, gc, st);
+    }, 9);
+    if (ms < best->ms) {
+      best->ms = ms;
+      best->algo = 4;
+      best->cfg = cfg;
+    }
+  }
+  (void)hipStreamSynchronize(st);
+  (void)hipStreamDestroy(st);
+  return best->algo ? TRID_STATUS_SUCCESS : TRID_STATUS_EXECUTION_FAILED;
+}
+
+template <typename REAL>
+bool adjoint_tuned_config(int dev, const Geom &g, bool grads, HybridCfg *cfg) {
+  const Key k = key_of(dev, KIND_ADJOINT, sizeof(REAL) == 8, grads ? 1 : 0, g);
+  Choice c;
+  if (!lookup(k, &c) && autotune_mode() == 1 && !capturing(current_stream())) {
+    if (tune_adjoint<REAL>(dev, g, grads, &c) == TRID_STATUS_SUCCESS) store(k, c);
+    else c.algo = 0;
+  } else if (!lookup(k, &c)) {
+    c.algo = 0;
+  }
+  if (c.algo == 4 && adjoint_cfg_valid<REAL>(g, c.cfg)) {
+    *cfg = c.cfg;
+    return true;
+  }
+  return adjoint_default_cfg<REAL>(g, cfg);
+}
+
 template <typename REAL>
 void ragged_tuned_config(int dev, int64_t max_len, int *M, int *per_cu, int *launches) {
   Geom g{max_len, 1, 0, 0, 0, 0};
@@ -535,6 +619,8 @@ template bool tuned_config<double, false>(int, const Geom &, HybridCfg *);
 template bool tuned_config<double, true>(int, const Geom &, HybridCfg *);
 template void ragged_tuned_config<float>(int, int64_t, int *, int *, int *);
 template void ragged_tuned_config<double>(int, int64_t, int *, int *, int *);
+template bool adjoint_tuned_config<float>(int, const Geom &, bool, HybridCfg *);
+template bool adjoint_tuned_config<double>(int, const Geom &, bool, HybridCfg *);
 
 }  // namespace trid
 
@@ -595,6 +681,24 @@ EXTERN_C tridStatus_t tridHipAutotuneMixedF32(int ndim, int solvedim, const int 
   Choice c;
   if ((s = tune_mixed32(dev, g, &c)) != TRID_STATUS_SUCCESS) return s;
   store(key_of(dev, KIND_MIXED32, 0, 0, g), c);
+  if (best_ms) *best_ms = c.ms;
+  return TRID_STATUS_SUCCESS;
+}
+
+EXTERN_C tridStatus_t tridHipAutotuneAdjoint(int fp64, int grads, int ndim, int solvedim,
+                                             const int *dims, const int *pads, float *best_ms) {
+  Geom g;
+  bool empty = false;
+  tridStatus_t s = make_geom(ndim, solvedim, dims, pads, &g, &empty);
+  if (s != TRID_STATUS_SUCCESS) return s;
+  int dev = 0;
+  if ((s = check_device(&dev)) != TRID_STATUS_SUCCESS) return s;
+  if (empty) return TRID_STATUS_SUCCESS;
+  Choice c;
+  s = fp64 ? tune_adjoint<double>(dev, g, grads != 0, &c)
+           : tune_adjoint<float>(dev, g, grads != 0, &c);
+  if (s != TRID_STATUS_SUCCESS) return s;
+  store(key_of(dev, KIND_ADJOINT, fp64 ? 1 : 0, grads ? 1 : 0, g), c);
   if (best_ms) *best_ms = c.ms;
   return TRID_STATUS_SUCCESS;
 }
