@@ -758,11 +758,19 @@ def main():
         log("fp64 %s: %s" % (name, dirs[name]))
     dom = max(per_dir_ms, key=per_dir_ms.get)
     ach = bytes_per_pt * n3 / (per_dir_ms[dom] * 1e-3) / 1e9
+    # HBM bytes per launch from the committed counter profile, reported only
+    # when this run's tuned shape is the one the counters were taken with
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("fp64_%d_%s" % (n, dom))
+            tj = json.load(f)
+        key = "fp64_%d_%s" % (n, dom)
+        if list(tj.get("configs", {}).get(key, [])) == list(cfgs[dom]):
+            traffic = tj.get(key)
+        else:
+            log("traffic: the profiled shape of %s differs from this run's %s; not reported"
+                % (key, cfgs[dom]))
 
     extras = {}
     if not args.quick:
