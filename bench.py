@@ -56,7 +56,7 @@ def parse():
                     help="directions of --only access")
     ap.add_argument("--only", choices=["fp32", "mixed", "mixed-f32", "adjoint", "ragged", "adi",
                                        "partitioned", "cpu-extras", "oracle-checks", "access",
-                                       "copy"],
+                                       "copy", "long"],
                     help="run one secondary workload and print its JSON instead of the bench "
                          "line (for profiling)")
     return ap.parse_args()
@@ -596,6 +596,26 @@ def bench_ragged(torch, api, tune):
     return out
 
 
+def bench_long(torch, api, tune):
+    """Systems longer than one workgroup holds (16384 FP64 rows; at most 8192
+    fit the hybrid), which take the per-lane Thomas kernel: 8192 systems
+    along x (contiguous: each lane walks its own system) and along y
+    (strided: neighbouring lanes read neighbouring addresses). 40 B per point
+    algorithmic; the kernel also writes and reads back c' (72 B moved)."""
+    out = {}
+    for name, sd, dims in (("x", 0, [16384, 8192, 1]), ("y", 1, [8192, 16384, 1])):
+        n = dims[0] * dims[1]
+        a, b, c, d = gen_gpu(torch, n, torch.float64, seed=16384)
+        if tune:
+            api.autotune(torch.float64, sd, dims)
+        ms = event_ms(torch, lambda: api.trid_strided_batch(a, b, c, d, solvedim=sd, dims=dims))
+        out[name] = {"dims": dims, "ms": round(ms, 4), "gpts_s": round(n / ms / 1e6, 2),
+                     "alg_GBs": round(40.0 * n / ms / 1e6, 1), "config": api.last_config()}
+        del a, b, c, d
+    torch.cuda.empty_cache()
+    return out
+
+
 def bench_partitioned(torch, api, n=512, shards=8):
     """One GPU's share when each z system of an n^3 FP64 grid is split into
     `shards` z-slabs (n x n x n/shards): the boundary step, the finishing
@@ -696,6 +716,7 @@ def main():
                    "ragged": lambda: bench_ragged(torch, api, tune),
                    "adi": lambda: bench_adi(torch),
                    "partitioned": lambda: bench_partitioned(torch, api),
+                   "long": lambda: bench_long(torch, api, tune),
                    "cpu-extras": cpu_extras,
                    "oracle-checks": lambda: oracle_checks(torch, api, n),
                    "access": lambda: bench_access(torch, api, tune, args.access_dirs, n),
@@ -786,6 +807,8 @@ def main():
         log("ragged: %s" % extras["ragged_40k"])
         extras["adi_256_graph"] = bench_adi(torch)
         log("adi: %s" % extras["adi_256_graph"])
+        extras["long_systems_16384_fp64"] = bench_long(torch, api, tune)
+        log("long systems: %s" % extras["long_systems_16384_fp64"])
         extras["partitioned_z_512_8_shards"] = bench_partitioned(torch, api)
         log("partitioned: %s" % extras["partitioned_z_512_8_shards"])
 

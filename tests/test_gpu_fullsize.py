@@ -85,3 +85,48 @@ def test_inc_equals_plain_plus_u_512cube():
         api.trid_strided_batch(a, b, c, d, solvedim=sd, dims=[N, N, N], u=u, sync=True)
         assert float((u - (u0 + x)).abs().max()) <= 1e-15 * float(u.abs().max())
         del x, u
+
+
+def test_more_than_2_pow_32_work_items():
+    """2^31 - 1 systems of 2 rows along x (FP32, 16 GiB per array): the
+    default shape (one row per lane, 128 systems per 256-thread workgroup)
+    needs 2^32 work-items, more than one HIP grid holds, so the launch is
+    split into sub-batches. Every system is checked against the 2 x 2
+    closed form on a sample of 200,000 systems spread over the whole batch
+    (first and last included), and the residual on all of them."""
+    ns = 2 ** 31 - 1
+    n = 2 * ns
+    g = torch.Generator(device="cuda")
+    g.manual_seed(2)
+    a = torch.rand(n, dtype=torch.float32, device="cuda", generator=g) * 2 - 1
+    c = torch.rand(n, dtype=torch.float32, device="cuda", generator=g) * 2 - 1
+    b = 2 + a.abs() + c.abs()
+    d = torch.rand(n, dtype=torch.float32, device="cuda", generator=g) * 2 - 1
+    x = d.clone()
+    api.trid_strided_batch(a, b, c, x, solvedim=0, dims=[2, ns], sync=True)
+    assert api.last_config()[:4] == (3, 1, 2, 128)
+    # residual of both rows of every system (a[0] and c[1] unused), in FP64 by chunks
+    worst = 0.0
+    step = 1 << 28
+    for s0 in range(0, ns, step):
+        s1 = min(ns, s0 + step)
+        sl = slice(2 * s0, 2 * s1)
+        av, bv, cv, dv, xv = (t[sl].view(-1, 2).double() for t in (a, b, c, d, x))
+        r0 = dv[:, 0] - bv[:, 0] * xv[:, 0] - cv[:, 0] * xv[:, 1]
+        r1 = dv[:, 1] - av[:, 1] * xv[:, 0] - bv[:, 1] * xv[:, 1]
+        worst = max(worst, float(torch.maximum(r0.abs(), r1.abs()).max()))
+        del av, bv, cv, dv, xv, r0, r1
+    assert worst <= 1e-5, worst
+    # closed form on a sample
+    idx = torch.cat([torch.tensor([0, ns - 1], device="cuda"),
+                     torch.randint(0, ns, (200000,), device="cuda", generator=g)])
+    a1, b0, b1, c0, d0, d1 = (t[2 * idx + k].double() for t, k in
+                              ((a, 1), (b, 0), (b, 1), (c, 0), (d, 0), (d, 1)))
+    det = b0 * b1 - c0 * a1
+    x0 = (d0 * b1 - c0 * d1) / det
+    x1 = (b0 * d1 - a1 * d0) / det
+    got0, got1 = x[2 * idx].double(), x[2 * idx + 1].double()
+    err = float(torch.maximum((got0 - x0).abs(), (got1 - x1).abs()).max())
+    assert err <= 1e-5 * float(torch.maximum(x0.abs(), x1.abs()).max()), err
+    del a, b, c, d, x
+    torch.cuda.empty_cache()
