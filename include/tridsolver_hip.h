@@ -36,7 +36,9 @@ EXTERN_C tridStatus_t tridHipGetStream(void **stream);
 EXTERN_C tridStatus_t tridHipReserveWorkspace(size_t bytes);
 /* Frees every workspace of the current device (synchronises the device). */
 EXTERN_C tridStatus_t tridHipReleaseWorkspaces(void);
-/* Workspace bytes the per-lane path needs for one call. */
+/* Workspace bytes one call may need: the per-lane path's, or, for systems
+ * longer than one workgroup holds (above 8192 contiguous / 2048 strided FP64
+ * rows), the larger of that and the long-system path's. */
 EXTERN_C size_t tridHipWorkspaceBytes(int fp64, int inc, int ndim, int solvedim,
                                       const int *dims);
 
@@ -47,7 +49,8 @@ EXTERN_C size_t tridHipWorkspaceBytes(int fp64, int inc, int ndim, int solvedim,
  * 2 mixed precision, 3 hybrid, 4 adjoint, 5 ragged, 6 streamed strided
  * hybrid, 7 streamed strided hybrid with LDS storage, 8 wide strided hybrid,
  * 9 ragged mixed precision, 10 FP64 solution of FP32-stored systems, 11 the
- * same for a ragged batch), and
+ * same for a ragged batch, 12 systems longer than one workgroup holds: the
+ * hybrid with its reduced system solved by a second batched solve), and
  * for the hybrid its
  * rows per lane M, lanes per system P, systems per workgroup W and XCD
  * grouping G (consecutive system groups dealt to one XCD; 1 = plain order).

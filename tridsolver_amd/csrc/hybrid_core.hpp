@@ -26,11 +26,14 @@ namespace hyb {
 // that group q of G gets a contiguous range of system groups makes
 // neighbouring groups -- which read neighbouring 128-byte row segments -- run
 // on one XCD at the same time.
-__device__ __forceinline__ int64_t xcd_remap(int64_t bid, int64_t nb, int G) {
+// Grid sizes and ids are 32-bit, so is this arithmetic (a 64-bit division
+// costs several times the scalar instructions of a 32-bit one).
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t nb, int G) {
   if (G <= 1) return bid;
-  const int64_t per = nb / G;
-  if (bid >= per * G) return bid;  // tail beyond a multiple of G: identity
-  return (bid % G) * per + bid / G;
+  const uint32_t g = (uint32_t)G;
+  const uint32_t per = nb / g;
+  if (bid >= per * g) return bid;  // tail beyond a multiple of G: identity
+  return (bid % g) * per + bid / g;
 }
 
 // Skewed tile index of element e of one system: one pad slot per M elements,
@@ -66,9 +69,10 @@ __device__ __forceinline__ Lane lane_of(const Geom &g, int P, int W, int G) {
   L.p = CONTIG ? L.t % P : L.t / W;
   L.w = CONTIG ? L.t / P : L.t % W;
   L.pstep = CONTIG ? 1 : W;
-  const int64_t nb0 = (g.n0 + W - 1) / W;
-  const int64_t bid = xcd_remap(blockIdx.x, gridDim.x, G);
-  const int64_t i0 = (bid % nb0) * W + L.w;
+  // n0 < 2^31 and one workgroup per W systems: 32-bit block arithmetic
+  const uint32_t nb0 = (uint32_t)((g.n0 + W - 1) / W);
+  const uint32_t bid = xcd_remap(blockIdx.x, gridDim.x, G);
+  const int64_t i0 = (int64_t)(bid % nb0) * W + L.w;
   const int64_t i1 = bid / nb0;
   L.active = i0 < g.n0;
   L.base = i0 * g.st0 + i1 * g.st1;

@@ -251,6 +251,98 @@ __global__ __launch_bounds__((AdiBounds<REAL, M>::threads)) void adi_solve_kerne
   }
 }
 
+// The matrix-free x sweep with every access in chunk order (x lines of
+// N = P * 8 rows, P lanes of 8 rows, one wave per workgroup): lane p loads
+// its own rows 8p .. 8p+7 of u and of the four neighbour lines as 16-byte
+// vectors, takes the x neighbours of its first and last row from lanes p-1
+// and p+1 by ds_bpermute, solves (the PCR exchanges are wave-local too) and
+// stores its rows as vectors. No LDS tile and no transpose: the values and
+// the arithmetic are those of adi_solve_kernel's x sweep, so the results are
+// the same bit for bit. For x lines whose rows start 16-byte aligned
+// (nx * sizeof(REAL) a multiple of 16) on grids below 4 GiB.
+constexpr int ADI_DX_M = 8;
+
+template <typename REAL>
+__global__ __launch_bounds__(64) void adi_x_direct_kernel(Geom g, int P, int W, int G,
+                                                          AdiCoef<REAL> kc,
+                                                          REAL *__restrict__ d,
+                                                          const REAL *__restrict__ u) {
+  constexpr int M = ADI_DX_M;
+  constexpr int V = 16 / (int)sizeof(REAL);  // elements per 16-byte vector
+  typedef REAL vec __attribute__((ext_vector_type(V)));
+  __shared__ __align__(16) unsigned char ring_lds[2 * 3 * sizeof(REAL) * 64];
+  const hyb::Lane L = hyb::lane_of<true>(g, P, W, G);
+  hyb::Ring ring{ring_lds, (int)(3 * sizeof(REAL) * 64), 0};
+  const int j = (int)L.i0, k = (int)L.i1;  // the x line (j, k)
+  const bool line_inner = L.active && j > 0 && j < kc.ny - 1 && k > 0 && k < kc.nz - 1;
+  const uint32_t sy = line_inner ? (uint32_t)(kc.nx * sizeof(REAL)) : 0u;
+  const uint32_t sz = line_inner ? (uint32_t)((uint32_t)kc.nx * kc.ny * sizeof(REAL)) : 0u;
+  // lanes past the last line read line 0; their rows are identity rows
+  const uint32_t o0 = ((L.active ? (uint32_t)L.base : 0u) + (uint32_t)(L.p * M)) *
+                      (uint32_t)sizeof(REAL);
+  const char *ub = reinterpret_cast<const char *>(u);
+  auto ldv = [ub](uint32_t off) { return *reinterpret_cast<const vec *>(ub + off); };
+  REAL uc[M], ym[M], yp[M], zm[M], zp[M];
+#pragma unroll
+  for (int q = 0; q < M / V; ++q) {
+    const uint32_t o = o0 + (uint32_t)(q * 16);
+    const vec c0 = ldv(o), c1 = ldv(o - sy), c2 = ldv(o + sy), c3 = ldv(o - sz), c4 = ldv(o + sz);
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+      uc[q * V + e] = c0[e];
+      ym[q * V + e] = c1[e];
+      yp[q * V + e] = c2[e];
+      zm[q * V + e] = c3[e];
+      zp[q * V + e] = c4[e];
+    }
+  }
+  // x neighbours: rows 8p-1 and 8p+8 live in lanes p-1 and p+1
+  REAL edge[2] = {uc[M - 1], uc[0]}, nb_lo[2], nb_hi[2];
+  hyb::exchange<3, 2>(L, ring, true, edge, L.pstep, L.p > 0, nb_lo, L.pstep, L.p + 1 < L.P,
+                      nb_hi);
+  REAL rd[M];
+#pragma unroll
+  for (int r = 0; r < M; ++r) {
+    const int row = L.p * M + r;
+    const bool inner = line_inner && row > 0 && row < L.N - 1;
+    const REAL xm = r > 0 ? uc[r - 1] : nb_lo[0];
+    const REAL xp = r < M - 1 ? uc[r + 1] : nb_hi[1];
+    rd[r] = inner ? adi_rhs(kc.lambda, xm, xp, ym[r], yp[r], zm[r], zp[r], uc[r]) : REAL(0);
+  }
+  REAL ra[M], rb[M], rc[M];
+  adi_coefs<REAL, M, true>(L, kc, ra, rb, rc);
+  REAL x[M];
+  hyb::solve_chunk<REAL, M, true>(L, ring, ra, rb, rc, rd, x);
+  if (!L.active) return;
+  char *db = reinterpret_cast<char *>(d);
+#pragma unroll
+  for (int q = 0; q < M / V; ++q) {
+    vec v;
+#pragma unroll
+    for (int e = 0; e < V; ++e) v[e] = x[q * V + e];
+    *reinterpret_cast<vec *>(db + o0 + (uint32_t)(q * 16)) = v;
+  }
+}
+
+// The direct x sweep applies: 8 rows per lane filling the lines exactly, a
+// whole line's lanes in the wave, 16-byte aligned rows and arrays.
+template <typename REAL>
+bool adi_x_direct_ok(const Geom &g, const HybridCfg &cfg, const REAL *d, const REAL *u,
+                     int nx) {
+  auto al16 = [](const void *q) { return (reinterpret_cast<uintptr_t>(q) & 15) == 0; };
+  return cfg.M == ADI_DX_M && (int64_t)cfg.P * ADI_DX_M == g.N && cfg.W * cfg.P <= 64 &&
+         ((int64_t)nx * (int64_t)sizeof(REAL)) % 16 == 0 && al16(d) && al16(u);
+}
+
+template <typename REAL>
+hipError_t launch_adi_x_direct(const Geom &g, const HybridCfg &cfg, const AdiCoef<REAL> &kc,
+                               REAL *d, const REAL *u, hipStream_t st) {
+  const int64_t blocks = (g.n0 + cfg.W - 1) / cfg.W * g.n1;
+  hipLaunchKernelGGL(adi_x_direct_kernel<REAL>, dim3((unsigned)blocks), dim3(cfg.W * cfg.P), 0,
+                     st, g, cfg.P, cfg.W, cfg.G, kc, d, u);
+  return hipGetLastError();
+}
+
 template <typename REAL, int M, bool INC, bool CONTIG, bool RHS = false>
 hipError_t launch_adi_solve_m(const Geom &g, const HybridCfg &cfg, const AdiCoef<REAL> &kc,
                               REAL *d, REAL *u, hipStream_t st) {
@@ -381,6 +473,14 @@ tridStatus_t adi_sweep(int dev, REAL lambda, REAL *d, REAL *u, int nx, int ny, i
     tridStatus_t sp = preproc<REAL>(lambda, u, d, nullptr, nullptr, nullptr, nullptr, nullptr,
                                     nullptr, nullptr, nullptr, nullptr, nx, ny, nz);
     if (sp != TRID_STATUS_SUCCESS) return sp;
+  }
+  if (contig && rhs_here && adi_x_direct_ok<REAL>(g, cfg, d, u, nx)) {
+    e = launch_adi_x_direct<REAL>(g, cfg, kc, d, u, st);
+    if (e != hipSuccess) {
+      record_hip_error(e);
+      return TRID_STATUS_EXECUTION_FAILED;
+    }
+    return TRID_STATUS_SUCCESS;
   }
 #define TRID_ADI_CASE(MM)                                                                  \
   case MM:                                                                                 \

@@ -178,7 +178,7 @@ tridStatus_t solve(const TridParams *ctx, const REAL *a, const REAL *b, const RE
 
   const int hint = ctx ? ctx->opts[solvedim] : 0;
   HybridCfg cfg;
-  bool use_hybrid = false;
+  bool use_hybrid = false, use_long = false;
   if (tl_forced.M != 0) {
     cfg = tl_forced;
     if (!hybrid_cfg_valid<REAL>(g, cfg)) return TRID_STATUS_INVALID_VALUE;
@@ -186,7 +186,10 @@ tridStatus_t solve(const TridParams *ctx, const REAL *a, const REAL *b, const RE
   } else if (hint == 1 || hint == 2) {
     use_hybrid = false;
   } else {
-    use_hybrid = tuned_config<REAL, INC>(dev, g, &cfg);
+    // 12: longer than one workgroup holds, the hybrid with an outside reduced solve
+    const int algo = tuned_solve_algo<REAL, INC>(dev, g, &cfg);
+    use_hybrid = algo == 3;
+    use_long = algo == 12;
   }
 
   if (use_hybrid) {
@@ -195,6 +198,13 @@ tridStatus_t solve(const TridParams *ctx, const REAL *a, const REAL *b, const RE
                                                [&](const Geom &sub, int64_t o) {
       return launch_hybrid<REAL, INC>(sub, cfg, a + o, b + o, c + o, d + o, advance(u, o), st);
     }));
+  }
+  if (use_long) {
+    record_config(12, nullptr);
+    const size_t need = long_workspace_bytes<REAL>(g);
+    void *lws = nullptr;
+    if ((s = get_workspace(dev, st, need, &lws)) != TRID_STATUS_SUCCESS) return s;
+    return finish(ctx, st, launch_long<REAL, INC>(dev, g, a, b, c, d, u, lws, need, st));
   }
   record_config(1, nullptr);
   const size_t one = sizeof(REAL) * (size_t)g.nsys() * (size_t)g.N;
@@ -219,7 +229,11 @@ size_t lane_workspace_need(int dev, const TridParams *ctx, int ndim, int solvedi
   if (tl_forced.M != 0) return 0;
   const int hint = ctx ? ctx->opts[solvedim] : 0;
   HybridCfg cfg;
-  if (hint != 1 && hint != 2 && tuned_config<REAL, INC>(dev, g, &cfg)) return 0;
+  if (hint != 1 && hint != 2) {
+    const int algo = tuned_solve_algo<REAL, INC>(dev, g, &cfg);
+    if (algo == 3) return 0;
+    if (algo == 12) return long_workspace_bytes<REAL>(g);
+  }
   const size_t one = sizeof(REAL) * (size_t)g.nsys() * (size_t)g.N;
   return INC ? 2 * one : one;
 }
@@ -342,7 +356,14 @@ EXTERN_C size_t tridHipWorkspaceBytes(int fp64, int inc, int ndim, int solvedim,
   if (make_geom(ndim, solvedim, dims, dims, &g, &empty) != TRID_STATUS_SUCCESS || empty)
     return 0;
   const size_t one = (fp64 ? 8 : 4) * (size_t)g.nsys() * (size_t)g.N;
-  return inc ? 2 * one : one;
+  const size_t lane = inc ? 2 * one : one;
+  // systems longer than one workgroup holds take the long-system path by
+  // default, with its own workspace
+  HybridCfg cfg;
+  const bool fits = fp64 ? hybrid_default_cfg<double>(g, &cfg) : hybrid_default_cfg<float>(g, &cfg);
+  if (fits) return lane;
+  const size_t lng = fp64 ? long_workspace_bytes<double>(g) : long_workspace_bytes<float>(g);
+  return std::max(lane, lng);
 }
 
 EXTERN_C tridStatus_t tridHipLastConfig(int *algorithm, int *M, int *P, int *W, int *G) {

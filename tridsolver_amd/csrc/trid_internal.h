@@ -81,6 +81,15 @@ size_t lane_workspace_need(int dev, const TridParams *ctx, int ndim, int solvedi
                            const int *dims, const int *pads);
 void drop_stream_workspaces(int dev, hipStream_t st);
 
+// ---- systems longer than one workgroup holds (kernels_long.hip) -----------
+// The hybrid with its reduced system solved by a second batched solve;
+// workspace: long_workspace_bytes(g) bytes.
+template <typename REAL>
+size_t long_workspace_bytes(const Geom &g);
+template <typename REAL, bool INC>
+hipError_t launch_long(int dev, const Geom &g, const REAL *a, const REAL *b, const REAL *c,
+                       REAL *d, REAL *u, void *ws, size_t ws_bytes, hipStream_t st);
+
 // ---- streamed strided variant (kernels_stream.hip) ------------------------
 template <typename REAL>
 bool stream_cfg_valid(const Geom &g, const HybridCfg &c);

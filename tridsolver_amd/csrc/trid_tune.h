@@ -5,8 +5,12 @@
 
 namespace trid {
 
-// Fills *cfg with the shape to use for g on device dev and returns true, or
-// returns false when the per-lane Thomas path should run instead.
+// The algorithm for g on device dev: 3 the hybrid (*cfg filled), 1 the
+// per-lane Thomas kernel, 12 the long-system path (systems longer than one
+// workgroup holds). A cached tuning result decides, else the defaults.
+template <typename REAL, bool INC>
+int tuned_solve_algo(int dev, const Geom &g, HybridCfg *cfg);
+// tuned_solve_algo(...) == 3.
 template <typename REAL, bool INC>
 bool tuned_config(int dev, const Geom &g, HybridCfg *cfg);
 

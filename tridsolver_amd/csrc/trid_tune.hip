@@ -253,6 +253,26 @@ tridStatus_t tune_solve(int dev, const Geom &g, Choice *best) {
       best->cfg = cfg;
     }
   }
+  // systems longer than one workgroup holds: the long-system path
+  {
+    HybridCfg one;
+    const size_t lwb = long_workspace_bytes<REAL>(g);
+    void *lws = nullptr;
+    if (!hybrid_default_cfg<REAL>(g, &one) && lwb < (size_t(16) << 30) &&
+        hipMalloc(&lws, lwb) == hipSuccess) {
+      const float ms = time_on(st, [&] {
+        return launch_long<REAL, INC>(dev, g, a, b, c, d, u, lws, lwb, st);
+      });
+      if (ms < best->ms) {
+        best->ms = ms;
+        best->algo = 12;
+      }
+      (void)hipStreamSynchronize(st);
+      (void)hipFree(lws);
+    } else {
+      (void)hipGetLastError();
+    }
+  }
   // per-lane Thomas, when its workspace fits comfortably
   const size_t wsb = sizeof(REAL) * (size_t)g.nsys() * (size_t)g.N * (INC ? 2 : 1);
   void *ws = nullptr;
@@ -406,7 +426,7 @@ tridStatus_t tune_mixed32(int dev, const Geom &g, Choice *best) {
 }  // namespace
 
 template <typename REAL, bool INC>
-bool tuned_config(int dev, const Geom &g, HybridCfg *cfg) {
+int tuned_solve_algo(int dev, const Geom &g, HybridCfg *cfg) {
   const Key k = key_of(dev, KIND_SOLVE, sizeof(REAL) == 8, INC, g);
   Choice c;
   if (!lookup(k, &c) && autotune_mode() == 1 && !capturing(current_stream())) {
@@ -417,10 +437,15 @@ bool tuned_config(int dev, const Geom &g, HybridCfg *cfg) {
   }
   if (c.algo == 3 && hybrid_cfg_valid<REAL>(g, c.cfg)) {
     *cfg = c.cfg;
-    return true;
+    return 3;
   }
-  if (c.algo == 1) return false;
-  return hybrid_default_cfg<REAL>(g, cfg);
+  if (c.algo == 1 || c.algo == 12) return c.algo;
+  return hybrid_default_cfg<REAL>(g, cfg) ? 3 : 12;
+}
+
+template <typename REAL, bool INC>
+bool tuned_config(int dev, const Geom &g, HybridCfg *cfg) {
+  return tuned_solve_algo<REAL, INC>(dev, g, cfg) == 3;
 }
 
 bool mixed_tuned_config(int dev, const Geom &g, HybridCfg *cfg) {
@@ -613,6 +638,10 @@ tridStatus_t tune_ragged(int dev, const int64_t *offsets, int64_t nsys, int64_t 
   return TRID_STATUS_SUCCESS;
 }
 
+template int tuned_solve_algo<float, false>(int, const Geom &, HybridCfg *);
+template int tuned_solve_algo<float, true>(int, const Geom &, HybridCfg *);
+template int tuned_solve_algo<double, false>(int, const Geom &, HybridCfg *);
+template int tuned_solve_algo<double, true>(int, const Geom &, HybridCfg *);
 template bool tuned_config<float, false>(int, const Geom &, HybridCfg *);
 template bool tuned_config<float, true>(int, const Geom &, HybridCfg *);
 template bool tuned_config<double, false>(int, const Geom &, HybridCfg *);
