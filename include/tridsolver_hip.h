@@ -90,7 +90,9 @@ EXTERN_C tridStatus_t tridHipAutotune(int fp64, int inc, int ndim, int solvedim,
 /* Tunes tridDmtsvStridedBatchMixed (2 refinements) for one geometry. */
 EXTERN_C tridStatus_t tridHipAutotuneMixed(int ndim, int solvedim, const int *dims,
                                            const int *pads, float *best_ms);
-/* Tunes tridDmtsvStridedBatchMixedF32 (1 refinement) for one geometry. */
+/* Tunes tridDmtsvStridedBatchMixedF32 for one geometry: the shape for one
+ * refinement and the shape for more (timed at 2). *best_ms is the first's
+ * time. */
 EXTERN_C tridStatus_t tridHipAutotuneMixedF32(int ndim, int solvedim, const int *dims,
                                               const int *pads, float *best_ms);
 /* Tunes trid{D,S}mtsvStridedBatchAdjoint for one geometry (fp64: 1 double /

@@ -228,14 +228,22 @@ def test_mixed_f32_vs_long_double_oracle(dims):
         assert np.array_equal(td.cpu().numpy(), d)
 
 
-@pytest.mark.parametrize("shape", [(32, 16, 32, 8, 3), (16, 32, 32, 8, 3), (32, 16, 16, 8, 3),
+def _general_ok(shape):
+    """Shapes with a kernel for more than one refinement: the register kernel
+    at 8 rows per lane, the contiguous LDS kernel, the wide kernel at 16 rows
+    per lane with 64-byte rows."""
+    M, _, W, _, V = shape
+    return (V == 0 and M == 8) or V == 1 or (V == 3 and (M, W) == (16, 16))
+
+
+@pytest.mark.parametrize("shape", [(32, 16, 32, 8, 3), (32, 16, 16, 8, 3), (16, 32, 16, 8, 3),
                                    (16, 32, 16, 8, 0), (8, 64, 8, 8, 0)])
 def test_mixed_f32_forced_strided_shapes(shape):
     """Each strided kernel shape forced along y and z of a 40 x 30 x 500 grid
-    (partial system groups): the wide kernel at 32 and 16 rows per lane with
-    128-byte rows (the latter at 1024 threads) and at 32 rows per lane with
-    64-byte rows, the register kernel at 16 and 8; one and two refinements
-    (two take the general refinement loop)."""
+    (partial system groups): the wide kernel at 32 rows per lane with 128-
+    and 64-byte rows and at 16 with 64-byte rows, the register kernel at 16
+    and 8; one refinement, and two (the general refinement loop) where the
+    shape has a kernel for it -- elsewhere two are refused."""
     M, P, W, G, V = shape
     rng = np.random.default_rng(M * 7 + V)
     dims = [40, 30, 500]
@@ -252,10 +260,42 @@ def test_mixed_f32_forced_strided_shapes(shape):
         xl = oracle.batch_ld(a, b, c, d, ndim=3, solvedim=sd, dims=dims)
         with api.forced_hybrid(M, Pn, W, G, V):
             for refine in (1, 2):
+                if refine == 2 and not _general_ok(shape):
+                    with pytest.raises(Exception):
+                        api.mixed_f32_strided_batch(*t, solvedim=sd, dims=dims,
+                                                    refinements=refine, sync=True)
+                    continue
                 x = api.mixed_f32_strided_batch(*t, solvedim=sd, dims=dims, refinements=refine,
                                                 sync=True)
                 assert api.last_config()[1:4] == (M, Pn, W)
                 assert H.norm_rel_err(x.cpu().numpy(), xl) <= 1e-12, (sd, refine)
+
+
+@pytest.mark.parametrize("shape", [(16, 2, 1), (16, 4, 8), (8, 1, 8), (8, 8, 1)])
+@pytest.mark.parametrize("N", [512, 300, 700, 37])
+def test_mixed_f32_forced_contig_lds(shape, N):
+    """The contiguous kernel with a and c kept in LDS (variant 1), forced at
+    8 and 16 rows per lane, one-wave and wider workgroups, lengths that are
+    not multiples of M and systems of more than 64 lanes (the 9-step PCR),
+    at 0, 1 and 2 refinements, against the long-double oracle of the
+    promoted system."""
+    M, W, G = shape
+    dims = [N, 7, 3]
+    rng = np.random.default_rng(N * 3 + M + W)
+    n = int(np.prod(dims))
+    a, b, c, d = _f32_case(rng, n)
+    t = [dev(v) for v in (a, b, c, d)]
+    P = -(-N // M)
+    if W * P > 512:
+        W = max(1, 512 // P)
+    xl = oracle.batch_ld(a, b, c, d, ndim=3, solvedim=0, dims=dims)
+    with api.forced_hybrid(M, P, W, G, 1):
+        for refine, tol in ((0, 1e-5), (1, 1e-12), (2, 1e-12)):
+            x = api.mixed_f32_strided_batch(*t, solvedim=0, dims=dims, refinements=refine,
+                                            sync=True)
+            assert api.last_config() == (10, M, P, W, G)
+            assert H.norm_rel_err(x.cpu().numpy(), xl) <= tol, (refine, M, P, W)
+    assert np.array_equal(t[3].cpu().numpy(), d)
 
 
 def test_mixed_f32_padded_layout():

@@ -140,9 +140,13 @@ def test_forced_hybrid_shapes(M, W, G, dtype):
 @pytest.mark.parametrize("variant", [1, 2])
 def test_forced_streamed_variant(W, G, inc, variant, dtype):
     """The streamed strided variants (loads a few rows ahead of the sweep; 2:
-    one sweep value per row in LDS): every compiled M, y and z, lengths not
-    multiples of M, Inc."""
-    Ms = (8, 16) if dtype == np.float64 else (16, 32)
+    one sweep value per row in LDS): the compiled M (8 FP64, 16 FP32), y and
+    z, lengths not multiples of M, Inc; twice that M is refused."""
+    Ms = (8,) if dtype == np.float64 else (16,)
+    dims, sd = [19, 45, 3], 1
+    with api.forced_hybrid(2 * Ms[0], -(-dims[sd] // (2 * Ms[0])), W, G, variant):
+        with pytest.raises(Exception):
+            run_case(dims, sd, dtype, inc, (0, 0, 0), seed=1)
     for M in Ms:
         for dims, sd in (([19, 45, 3], 1), ([17, 4, 51], 2), ([40, 37, 9], 1), ([33, 8, 130], 2)):
             N = dims[sd]

@@ -68,9 +68,10 @@ def test_autotune_on_first_use():
 
 
 def test_autotune_mixed_f32():
-    """Tuning the FP32-stored solve picks a valid shape per direction (the
-    solve then reports it) and the result stays within 1e-12 of the
-    long-double oracle's solve of the promoted values."""
+    """Tuning the FP32-stored solve picks a valid shape per direction for one
+    refinement and one for more (the solve then reports it), and the results
+    at 1 and 2 refinements stay within 1e-12 of the long-double oracle's
+    solve of the promoted values."""
     dims = [128, 64, 32]
     rng = np.random.default_rng(21)
     n = int(np.prod(dims))
@@ -78,10 +79,12 @@ def test_autotune_mixed_f32():
     t = [dev(v) for v in (a, b, c, d)]
     for sd in range(3):
         assert api.autotune_mixed_f32(sd, dims) > 0
-        x = api.mixed_f32_strided_batch(*t, solvedim=sd, dims=dims, refinements=1, sync=True)
-        assert api.last_config()[0] == 10
         xl = oracle.batch_ld(a, b, c, d, ndim=3, solvedim=sd, dims=dims)
-        assert H.norm_rel_err(x.cpu().numpy(), xl) <= 1e-12
+        for refine in (1, 2):
+            x = api.mixed_f32_strided_batch(*t, solvedim=sd, dims=dims, refinements=refine,
+                                            sync=True)
+            assert api.last_config()[0] == 10
+            assert H.norm_rel_err(x.cpu().numpy(), xl) <= 1e-12, (sd, refine)
 
 
 def test_autotune_mixed_f32_save_load(tmp_path):
@@ -95,17 +98,21 @@ def test_autotune_mixed_f32_save_load(tmp_path):
     tuned = {}
     for sd in range(3):
         assert api.autotune_mixed_f32(sd, dims) > 0
-        api.mixed_f32_strided_batch(*t, solvedim=sd, dims=dims, refinements=1, sync=True)
-        tuned[sd] = api.last_config()
+        for refine in (1, 2):
+            api.mixed_f32_strided_batch(*t, solvedim=sd, dims=dims, refinements=refine,
+                                        sync=True)
+            tuned[sd, refine] = api.last_config()
     path = str(tmp_path / "tune_f32.txt").encode()
     assert _lib.lib().tridHipAutotuneSave(path) == 0
     assert _lib.lib().tridHipAutotuneClear() == 0
     assert _lib.lib().tridHipAutotuneLoad(path) == 0
     for sd in range(3):
-        x = api.mixed_f32_strided_batch(*t, solvedim=sd, dims=dims, refinements=1, sync=True)
-        assert api.last_config() == tuned[sd]
         xl = oracle.batch_ld(a, b, c, d, ndim=3, solvedim=sd, dims=dims)
-        assert H.norm_rel_err(x.cpu().numpy(), xl) <= 1e-12
+        for refine in (1, 2):
+            x = api.mixed_f32_strided_batch(*t, solvedim=sd, dims=dims, refinements=refine,
+                                            sync=True)
+            assert api.last_config() == tuned[sd, refine]
+            assert H.norm_rel_err(x.cpu().numpy(), xl) <= 1e-12
     _lib.lib().tridHipAutotuneClear()
 
 

@@ -114,25 +114,32 @@ __device__ __forceinline__ void fetch_abcd(const Lane &L, const TI *a, const TI 
   fetch<M, CONTIG>(L, d, rd, TR(0), 0, 0, L.N);
 }
 
+// dst = v for the lane's rows below N. The lane's base offset passes through
+// an empty asm: the row addresses are formed here, instead of the fetch's
+// being kept live (in registers) across the solve between the two.
 template <int M, bool CONTIG, typename TR, typename TO>
 __device__ __forceinline__ void store(const Lane &L, TO *__restrict__ dst, const TR (&v)[M]) {
   if (!L.active) return;
+  const int64_t row0 = fetch_row<M, CONTIG>(L, 0);
+  int64_t b = L.base + row0 * L.stride;
+  asm volatile("" : "+v"(b));
+  const int64_t step = CONTIG ? (int64_t)L.P : L.stride;
 #pragma unroll
-  for (int r = 0; r < M; ++r) {
-    const int64_t row = fetch_row<M, CONTIG>(L, r);
-    if (row < L.N) dst[L.base + row * L.stride] = (TO)v[r];
-  }
+  for (int r = 0; r < M; ++r)
+    if (fetch_row<M, CONTIG>(L, r) < L.N) dst[b + r * step] = (TO)v[r];
 }
 
 template <int M, bool CONTIG, typename TR, typename TO>
 __device__ __forceinline__ void store_add(const Lane &L, TO *__restrict__ dst,
                                           const TR (&v)[M]) {
   if (!L.active) return;
+  const int64_t row0 = fetch_row<M, CONTIG>(L, 0);
+  int64_t b = L.base + row0 * L.stride;
+  asm volatile("" : "+v"(b));
+  const int64_t step = CONTIG ? (int64_t)L.P : L.stride;
 #pragma unroll
-  for (int r = 0; r < M; ++r) {
-    const int64_t row = fetch_row<M, CONTIG>(L, r);
-    if (row < L.N) dst[L.base + row * L.stride] += (TO)v[r];
-  }
+  for (int r = 0; r < M; ++r)
+    if (fetch_row<M, CONTIG>(L, r) < L.N) dst[b + r * step] += (TO)v[r];
 }
 
 // ---- 16-byte vector access for contiguous systems --------------------------

@@ -13,17 +13,23 @@
 // against the 40 B the FP64 solve of the same system moves when it is
 // stored in FP64.
 //
-// Two kernels:
+// Three kernels:
 //   mixed32_kernel       every operand in registers; contiguous systems go
 //                        through an LDS tile per system (float operands in,
 //                        FP64 x out), strided ones take W x-neighbours.
+//   mixed32_lds_kernel   a and c kept in LDS (2*M registers fewer a lane):
+//                        contiguous systems in their transpose tile,
+//                        strided ones in per-lane slots.
 //   mixed32_wide_kernel  strided systems, 32 x-neighbours per workgroup, so
 //                        every row is one 128-byte segment (what keeps the
 //                        z solve's address translations per byte low), or 16
-//                        (64-byte rows, two workgroups per CU); `a`
-//                        and `c` go straight from HBM into LDS tiles
-//                        [row][system] by global->LDS loads (one dword per
-//                        lane: one FP32 element), b and d into registers.
+//                        (64-byte rows); `a` and `c` go straight from HBM
+//                        into LDS tiles [row][system] by global->LDS loads
+//                        (one dword per lane: one FP32 element), b and d
+//                        into registers.
+// More than one refinement keeps x in FP64 and d through the loop (the
+// general loop of mixed_core.hpp): only the shapes whose general kernel fits
+// the register file without spilling offer it (general_ok).
 
 #include <algorithm>
 #include <atomic>
@@ -65,12 +71,75 @@ __global__ __launch_bounds__(M32_THREADS) void mixed32_kernel(
   hyb::store<M, CONTIG>(L, x, X);
 }
 
+// a and c kept in LDS, read from there by the solve, which frees 2*M
+// registers a lane against mixed32_kernel. Contiguous systems: b and d go to
+// chunk order through the system's tile, then a and c are written into it
+// (skewed, the two FP32 halves of the P*(M+1)-double tile); x's transpose
+// reuses the whole tile once every lane has read a and c. Strided systems:
+// the lane's own a and c slots, [row][thread] (no exchange between lanes).
+// STEPS: PCR steps, P <= 2^STEPS.
+template <int M, int STEPS, bool ONE, bool CONTIG>
+__global__ __launch_bounds__(M32_THREADS) void mixed32_lds_kernel(
+    Geom g, int P, int W, int G, int refine, const float *__restrict__ a,
+    const float *__restrict__ b, const float *__restrict__ c, const float *__restrict__ d,
+    double *__restrict__ x) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const hyb::Lane L = hyb::lane_of<CONTIG>(g, P, W, G);
+  if constexpr (!CONTIG) {
+    const int T = L.T, t = L.t;
+    float *tA = reinterpret_cast<float *>(smem_raw), *tC = tA + (size_t)M * T;
+    hyb::Ring ring{smem_raw + 2 * sizeof(float) * (size_t)M * T, (int)(2 * sizeof(double) * T),
+                   0};
+    float A[M], B[M], Cc[M], D[M];
+    hyb::fetch_abcd<M, false>(L, a, b, c, d, A, B, Cc, D);
+#pragma unroll
+    for (int r = 0; r < M; ++r) {
+      tA[r * T + t] = A[r];
+      tC[r * T + t] = Cc[r];
+    }
+    // the solve reads a and c back from LDS; without this the compiler keeps
+    // the stored registers live instead
+    asm volatile("" ::: "memory");
+    double X[M];
+    mixed_chunk_acc<M, STEPS, false, ONE, true>(
+        L, ring, [&](int r) { return tA[r * T + t]; }, B, [&](int r) { return tC[r * T + t]; },
+        D, refine, X);
+    hyb::store<M, false>(L, x, X);
+    return;
+  }
+  const int tile = P * (M + 1);
+  double *dtile = reinterpret_cast<double *>(smem_raw) + (size_t)L.w * tile;
+  float *tA = reinterpret_cast<float *>(dtile), *tC = tA + tile;
+  hyb::Ring ring{smem_raw + sizeof(double) * (size_t)L.T * (M + 1),
+                 (int)(2 * sizeof(double) * L.T), 0};
+  {
+    float A[M], B[M], Cc[M], D[M];
+    hyb::fetch_abcd<M, true>(L, a, b, c, d, A, B, Cc, D);
+    hyb::to_chunk<M, true>(L, B, tA);
+    hyb::to_chunk<M, true>(L, D, tA);
+#pragma unroll
+    for (int r = 0; r < M; ++r) {
+      const int e = hyb::skew<M>(L.p + L.P * r);
+      tA[e] = A[r];
+      tC[e] = Cc[r];
+    }
+    __syncthreads();
+    const float *ca = tA + L.p * (M + 1), *cc = tC + L.p * (M + 1);
+    double X[M];
+    mixed_chunk_acc<M, STEPS, false, ONE, true>(
+        L, ring, [&](int r) { return ca[r]; }, B, [&](int r) { return cc[r]; }, D, refine, X);
+    __syncthreads();  // a and c are read to the end before x overwrites the tile
+    hyb::to_fetch<M, true>(L, X, dtile);
+    hyb::store<M, true>(L, x, X);
+  }
+}
+
 typedef __attribute__((address_space(3))) void lds_void;
 
 // Threads per workgroup of the wide kernel with W FP32 x-neighbours (W = 32:
 // 128-byte rows, W = 16: 64-byte rows): M = 32 rows per lane at 16 lanes
-// per system (two waves per SIMD; W = 16 fits two workgroups per CU), M = 16
-// at 32 lanes (W = 32: 1024 threads, four waves per SIMD).
+// per system (two waves per SIMD), M = 16 at 32 lanes. Built: (M, W) =
+// (32, 32), (32, 16) and (16, 16).
 template <int M, int W>
 struct M32Wide {
   static constexpr int threads = (M >= 32 ? 16 : 32) * W;
@@ -188,8 +257,8 @@ bool wide_valid(const Geom &g, const HybridCfg &c) {
   if (g.stride == 1 || g.st0 != 1) return false;
   int maxT = 0, steps = 0;
   if (c.W == 32 && c.M == 32) { maxT = M32Wide<32, 32>::threads; steps = M32Wide<32, 32>::steps; }
-  if (c.W == 32 && c.M == 16) { maxT = M32Wide<16, 32>::threads; steps = M32Wide<16, 32>::steps; }
   if (c.W == 16 && c.M == 32) { maxT = M32Wide<32, 16>::threads; steps = M32Wide<32, 16>::steps; }
+  if (c.W == 16 && c.M == 16) { maxT = M32Wide<16, 16>::threads; steps = M32Wide<16, 16>::steps; }
   if (maxT == 0 || c.P < 1 || c.G < 1) return false;
   const int64_t T = (int64_t)c.W * c.P;
   if ((int64_t)c.P * c.M < g.N || c.P > (1 << steps) || T > maxT || T % 64 != 0) return false;
@@ -198,6 +267,21 @@ bool wide_valid(const Geom &g, const HybridCfg &c) {
   const int64_t last = (nb0 - 1) + (g.n1 - 1) * g.st1 + ((int64_t)c.P * c.M - 1) * g.stride;
   if ((last + 1) * (int64_t)sizeof(double) > (int64_t)0xFFFFFFFFll) return false;
   return wide_lds_bytes(c) <= M32_LDS_MAX;
+}
+
+size_t lds_kernel_bytes(const Geom &g, const HybridCfg &c) {
+  const size_t T = (size_t)c.W * c.P;
+  const size_t tiles = g.stride == 1 ? sizeof(double) * T * (c.M + 1)  // per-system tiles
+                                     : 2 * sizeof(float) * T * c.M;    // a, c [row][thread]
+  return tiles + hyb::ring_bytes<double>((int)T, 2);
+}
+
+bool lds_valid(const Geom &g, const HybridCfg &c) {
+  if (c.M != 8 && c.M != 16) return false;
+  if (c.P < 1 || c.W < 1 || c.G < 1) return false;
+  if ((int64_t)c.P * c.M < g.N || c.P > (1 << MIXED_MAX_STEPS)) return false;
+  if ((int64_t)c.W * c.P > M32_THREADS) return false;
+  return lds_kernel_bytes(g, c) <= M32_LDS_MAX;
 }
 
 bool reg_valid(const Geom &g, const HybridCfg &c) {
@@ -231,8 +315,49 @@ template <int M, bool CONTIG>
 hipError_t launch_reg(const Geom &g, const HybridCfg &cfg, int refine, const float *a,
                       const float *b, const float *c, const float *d, double *x,
                       hipStream_t st) {
-  return refine <= 1 ? launch_reg_one<M, CONTIG, true>(g, cfg, refine, a, b, c, d, x, st)
-                     : launch_reg_one<M, CONTIG, false>(g, cfg, refine, a, b, c, d, x, st);
+  if (refine <= 1) return launch_reg_one<M, CONTIG, true>(g, cfg, refine, a, b, c, d, x, st);
+  if constexpr (M == 8)
+    return launch_reg_one<M, CONTIG, false>(g, cfg, refine, a, b, c, d, x, st);
+  return hipErrorInvalidValue;  // not general_ok
+}
+
+template <int M, int STEPS, bool ONE, bool CONTIG>
+hipError_t launch_lds_one(const Geom &g, const HybridCfg &cfg, int refine, const float *a,
+                          const float *b, const float *c, const float *d, double *x,
+                          hipStream_t st) {
+  const int T = cfg.W * cfg.P;
+  const size_t lds = lds_kernel_bytes(g, cfg);
+  auto kern = mixed32_lds_kernel<M, STEPS, ONE, CONTIG>;
+  static LdsGrant lds_granted;
+  {
+    hipError_t e = ensure_lds(reinterpret_cast<const void *>(kern), lds, lds_granted);
+    if (e != hipSuccess) return e;
+  }
+  const int64_t blocks = (g.n0 + cfg.W - 1) / cfg.W * g.n1;
+  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(T), lds, st, g, cfg.P, cfg.W, cfg.G,
+                     refine, a, b, c, d, x);
+  return hipGetLastError();
+}
+
+// Systems of at most 64 lanes take the 6-step PCR (fewer registers).
+template <int M, bool CONTIG>
+hipError_t launch_lds_c(const Geom &g, const HybridCfg &cfg, int refine, const float *a,
+                        const float *b, const float *c, const float *d, double *x,
+                        hipStream_t st) {
+  constexpr int S9 = MIXED_MAX_STEPS;
+  if (cfg.P <= 64)
+    return refine <= 1 ? launch_lds_one<M, 6, true, CONTIG>(g, cfg, refine, a, b, c, d, x, st)
+                       : launch_lds_one<M, 6, false, CONTIG>(g, cfg, refine, a, b, c, d, x, st);
+  return refine <= 1 ? launch_lds_one<M, S9, true, CONTIG>(g, cfg, refine, a, b, c, d, x, st)
+                     : launch_lds_one<M, S9, false, CONTIG>(g, cfg, refine, a, b, c, d, x, st);
+}
+
+template <int M>
+hipError_t launch_lds(const Geom &g, const HybridCfg &cfg, int refine, const float *a,
+                      const float *b, const float *c, const float *d, double *x,
+                      hipStream_t st) {
+  return g.stride == 1 ? launch_lds_c<M, true>(g, cfg, refine, a, b, c, d, x, st)
+                       : launch_lds_c<M, false>(g, cfg, refine, a, b, c, d, x, st);
 }
 
 template <int M, int W, bool ONE>
@@ -257,64 +382,84 @@ template <int M, int W>
 hipError_t launch_wide32(const Geom &g, const HybridCfg &cfg, int refine, const float *a,
                          const float *b, const float *c, const float *d, double *x,
                          hipStream_t st) {
-  return refine <= 1 ? launch_wide32_one<M, W, true>(g, cfg, refine, a, b, c, d, x, st)
-                     : launch_wide32_one<M, W, false>(g, cfg, refine, a, b, c, d, x, st);
+  if (refine <= 1) return launch_wide32_one<M, W, true>(g, cfg, refine, a, b, c, d, x, st);
+  if constexpr (M == 16 && W == 16)
+    return launch_wide32_one<M, W, false>(g, cfg, refine, a, b, c, d, x, st);
+  return hipErrorInvalidValue;  // not general_ok
+}
+
+// Shapes with a kernel for more than one refinement: the others' general
+// loop would spill (mixed32_kernel at M = 16, the wide kernel at M = 32) or
+// run at one wave per SIMD.
+bool general_ok(const HybridCfg &c) {
+  if (c.V == 0) return c.M == 8;
+  if (c.V == 1) return true;
+  if (c.V == 3) return c.M == 16 && c.W == 16;
+  return false;
 }
 
 }  // namespace
 
-// V = 0: mixed32_kernel (M = 8 or 16, W * P <= 512); V = 3: the wide
-// strided kernel (W = 32 with M = 16 or 32; W = 16 with M = 32).
-bool mixed32_cfg_valid(const Geom &g, const HybridCfg &c) {
+// V = 0: mixed32_kernel (M = 8 or 16, W * P <= 512); V = 1: the kernel
+// with a and c in LDS (M = 8 or 16, W * P <= 512); V = 3: the wide
+// strided kernel ((M, W) = (32, 32), (32, 16), (16, 16)). More than one
+// refinement: general_ok shapes only.
+bool mixed32_cfg_valid(const Geom &g, const HybridCfg &c, int refine) {
+  if (refine > 1 && !general_ok(c)) return false;
   if (c.V == 3) return wide_valid(g, c);
+  if (c.V == 1) return lds_valid(g, c);
   if (c.V != 0) return false;
   return reg_valid(g, c);
 }
 
-// Default shapes, from a sweep at 512^3 (one refinement): contiguous
-// systems 16 rows per lane from 256 rows on (else 8), in workgroups of up to
-// 256 threads (x: 0.67-0.69 ms, against 0.72-0.74 at 8 rows per lane, 0.78
-// at 4 and 0.85 at 512 threads; 16-byte vector loads measured 1.10-1.27 ms
-// and are not offered); strided systems the wide kernel at 32 rows per lane with the
-// workgroups dealt to the XCDs in 32 groups (y 0.68, z 0.81 ms; 16 rows per
-// lane in 1024 threads spill a few registers and run 0.82 / 1.15 ms), else
-// 16 x-neighbours in registers.
-// More than one refinement needs x in FP64 and d kept through the loop: the
-// shapes above spill there, so those calls take 8 rows per lane in registers.
+// Default shapes, from sweeps at 512^3 (times on one box; they vary by a few
+// percent between boxes):
+// - contiguous systems: the kernel with a and c in LDS at 8 rows per lane
+//   (16 past 4096 rows), workgroups of about one wave (W = 64 / P
+//   systems): x 0.61-0.63 ms at one refinement (122 VGPRs, four waves per
+//   SIMD; mixed32_kernel at 16 rows per lane 0.65-0.69, at 8 0.67-0.73),
+//   0.74 at two (mixed32_kernel at 8 rows per lane 1.14).
+// - strided systems, one refinement: the wide kernel at 32 rows per lane,
+//   128-byte rows, workgroups dealt to the XCDs in 32 groups (y 0.65, z
+//   0.80 ms), else 16 x-neighbours in registers. More: a and c in LDS, 8
+//   rows per lane (16 past 4096 rows), 8 x-neighbours (y 0.89, z 1.21 ms
+//   at two; the wide kernel at 16 rows per lane 0.99 / 1.28,
+//   mixed32_kernel 1.58 / 2.03).
 bool mixed32_default_cfg(const Geom &g, int refine, HybridCfg *cfg) {
   HybridCfg c;
-  c.G = 8;
-  if (refine > 1) {
-    c.M = 8;
-    c.V = 0;
-    c.P = (int)((g.N + c.M - 1) / c.M);
-    c.W = g.stride == 1 ? (int)std::max<int64_t>(1, 256 / c.P) : 8;
-    while (c.W > 1 && (int64_t)c.W * c.P > M32_THREADS) c.W /= 2;
-    if (reg_valid(g, c)) { *cfg = c; return true; }
-  }
-  if (g.stride != 1) {
-    for (int M : {32, 16}) {
+  const bool contig = g.stride == 1;
+  if (contig || refine > 1) {
+    for (int M : {8, 16}) {
       c.M = M;
-      c.V = 3;
-      c.G = 32;
-      c.W = 32;
+      c.V = 1;
       c.P = (int)((g.N + M - 1) / M);
-      c.P += c.P & 1;  // whole waves: 32 * P threads
-      if (wide_valid(g, c)) { *cfg = c; return true; }
+      if (contig) {
+        c.G = 1;
+        c.W = (int)std::max<int64_t>(1, 64 / c.P);
+      } else {
+        c.G = 8;
+        c.W = 8;
+        while (c.W > 1 && (int64_t)c.W * c.P > M32_THREADS) c.W /= 2;
+      }
+      if (mixed32_cfg_valid(g, c, refine)) { *cfg = c; return true; }
     }
+    return false;
   }
+  c.V = 3;
+  c.G = 32;
+  c.M = 32;
+  c.W = 32;
+  c.P = (int)((g.N + c.M - 1) / c.M);
+  c.P += c.P & 1;  // whole waves: W * P a multiple of 64
+  if (mixed32_cfg_valid(g, c, refine)) { *cfg = c; return true; }
+  c.V = 0;
   c.G = 8;
-  for (int M : {g.N >= 256 ? 16 : 8, 16}) {
+  for (int M : {g.N >= 256 ? 16 : 8, 8, 16}) {
     c.M = M;
-    c.V = 0;
     c.P = (int)((g.N + M - 1) / M);
-    if (g.stride == 1) {
-      c.W = (int)std::max<int64_t>(1, 256 / c.P);
-    } else {
-      c.W = 16;
-      while (c.W > 1 && (int64_t)c.W * c.P > M32_THREADS) c.W /= 2;
-    }
-    if (reg_valid(g, c)) { *cfg = c; return true; }
+    c.W = 16;
+    while (c.W > 1 && (int64_t)c.W * c.P > M32_THREADS) c.W /= 2;
+    if (mixed32_cfg_valid(g, c, refine)) { *cfg = c; return true; }
   }
   return false;
 }
@@ -324,12 +469,17 @@ hipError_t launch_mixed32(const Geom &g, const HybridCfg &cfg, int refine, const
                           hipStream_t st) {
   const bool contig = g.stride == 1;
   if (cfg.V == 3) {
-    if (cfg.W == 32 && cfg.M == 16)
-      return launch_wide32<16, 32>(g, cfg, refine, a, b, c, d, x, st);
     if (cfg.W == 32 && cfg.M == 32)
       return launch_wide32<32, 32>(g, cfg, refine, a, b, c, d, x, st);
     if (cfg.W == 16 && cfg.M == 32)
       return launch_wide32<32, 16>(g, cfg, refine, a, b, c, d, x, st);
+    if (cfg.W == 16 && cfg.M == 16)
+      return launch_wide32<16, 16>(g, cfg, refine, a, b, c, d, x, st);
+    return hipErrorInvalidValue;
+  }
+  if (cfg.V == 1) {
+    if (cfg.M == 8) return launch_lds<8>(g, cfg, refine, a, b, c, d, x, st);
+    if (cfg.M == 16) return launch_lds<16>(g, cfg, refine, a, b, c, d, x, st);
     return hipErrorInvalidValue;
   }
   switch (cfg.M) {

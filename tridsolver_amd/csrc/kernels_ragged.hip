@@ -350,12 +350,14 @@ __global__ __launch_bounds__(RG_MIXED_THREADS) void rg_solve_mixed(
 
 // FP64 solution of FP32-stored systems (the arithmetic of kernels_mixed32.hip):
 // FP32 a, b, c, d in, FP64 x out (d is not written); 16 rows per lane in
-// 512 threads, so systems up to 8192 rows; ONE: at most one refinement (the
-// general loop spills at this shape). The LDS tile holds doubles for x's
-// transpose; the FP32 operands go through its first half.
+// 512 threads, so systems up to 8192 rows; ONE: at most one refinement. The
+// LDS tile holds doubles for x's transpose; the FP32 operands go through its
+// first half. AC_LDS keeps a and c in the tile (its two FP32 halves) for the
+// solve, 32 registers a lane fewer: the general refinement loop needs that
+// to fit the register file.
 constexpr int RG_M32_M = 16;
 
-template <bool ONE>
+template <bool ONE, bool AC_LDS>
 __global__ __launch_bounds__(RG_MIXED_THREADS) void rg_solve_mixed32(
     const float *__restrict__ a, const float *__restrict__ b, const float *__restrict__ c,
     const float *__restrict__ d, double *__restrict__ x, const int64_t *__restrict__ off,
@@ -380,15 +382,30 @@ __global__ __launch_bounds__(RG_MIXED_THREADS) void rg_solve_mixed32(
                    (int)(2 * sizeof(double) * Tmax), 0};
     float A[M], B[M], Cc[M], D[M];
     hyb::fetch_abcd<M, true>(L, a, b, c, d, A, B, Cc, D);
-    hyb::to_chunk<M, true>(L, A, ftile);  // has barriers: s_next is visible below
+    hyb::to_chunk<M, true>(L, B, ftile);  // has barriers: s_next is visible below
     const int nxt = s_next;
     const RgMeta next = nxt < total ? rg_meta(plan, list, off, nxt, t, Tmax) : RgMeta{};
-    hyb::to_chunk<M, true>(L, B, ftile);
-    hyb::to_chunk<M, true>(L, Cc, ftile);
     hyb::to_chunk<M, true>(L, D, ftile);
     double X[M];
-    mixed_chunk_acc<M, MIXED_MAX_STEPS, false, ONE, true>(
-        L, ring, [&](int r) { return A[r]; }, B, [&](int r) { return Cc[r]; }, D, refine, X);
+    if constexpr (AC_LDS) {
+      float *tC = ftile + L.P * (M + 1);
+#pragma unroll
+      for (int r = 0; r < M; ++r) {
+        const int e = hyb::skew<M>(L.p + L.P * r);
+        ftile[e] = A[r];
+        tC[e] = Cc[r];
+      }
+      __syncthreads();
+      const float *ca = ftile + L.p * (M + 1), *cc = tC + L.p * (M + 1);
+      mixed_chunk_acc<M, MIXED_MAX_STEPS, false, ONE, true>(
+          L, ring, [&](int r) { return ca[r]; }, B, [&](int r) { return cc[r]; }, D, refine, X);
+      __syncthreads();  // a and c are read to the end before x overwrites the tile
+    } else {
+      hyb::to_chunk<M, true>(L, A, ftile);
+      hyb::to_chunk<M, true>(L, Cc, ftile);
+      mixed_chunk_acc<M, MIXED_MAX_STEPS, false, ONE, true>(
+          L, ring, [&](int r) { return A[r]; }, B, [&](int r) { return Cc[r]; }, D, refine, X);
+    }
     hyb::to_fetch<M, true>(L, X, dtile);
     hyb::store<M, true>(L, x, X);
     item = nxt;
@@ -396,7 +413,7 @@ __global__ __launch_bounds__(RG_MIXED_THREADS) void rg_solve_mixed32(
   }
 }
 
-template <bool ONE>
+template <bool ONE, bool AC_LDS>
 hipError_t launch_mixed32_one(const float *a, const float *b, const float *c, const float *d,
                               double *x, const int64_t *off, int64_t S, int per_cu, int refine,
                               void *ws, hipStream_t st, int dev) {
@@ -406,7 +423,7 @@ hipError_t launch_mixed32_one(const float *a, const float *b, const float *c, co
   launch_plan(off, S, RG_M32_M, Tmax, Tmax, ws, st);  // one launch: no long range
   const size_t lds =
       sizeof(double) * (size_t)Tmax * (RG_M32_M + 1) + hyb::ring_bytes<double>(Tmax, 2);
-  auto kern = rg_solve_mixed32<ONE>;
+  auto kern = rg_solve_mixed32<ONE, AC_LDS>;
   static LdsGrant lds_granted;
   {
     hipError_t e = ensure_lds(reinterpret_cast<const void *>(kern), lds, lds_granted);
@@ -540,8 +557,8 @@ hipError_t launch_ragged_mixed32(const float *a, const float *b, const float *c,
                                  double *x, const int64_t *off, int64_t S, int per_cu,
                                  int refine, void *ws, hipStream_t st, int dev) {
   return refine <= 1
-             ? launch_mixed32_one<true>(a, b, c, d, x, off, S, per_cu, refine, ws, st, dev)
-             : launch_mixed32_one<false>(a, b, c, d, x, off, S, per_cu, refine, ws, st, dev);
+             ? launch_mixed32_one<true, false>(a, b, c, d, x, off, S, per_cu, refine, ws, st, dev)
+             : launch_mixed32_one<false, true>(a, b, c, d, x, off, S, per_cu, refine, ws, st, dev);
 }
 
 int ragged_mixed_max_len(int M) {

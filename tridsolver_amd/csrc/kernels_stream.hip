@@ -227,8 +227,10 @@ hipError_t launch_stream_m(const Geom &g, const HybridCfg &cfg, const REAL *a, c
 
 }  // namespace
 
-// Rows per lane compiled for the streamed variant.
-bool stream_m_supported(int fp64, int M) { return fp64 ? (M == 8 || M == 16) : (M == 16 || M == 32); }
+// Rows per lane compiled for the streamed variants: 8 (FP64) and 16 (FP32).
+// Twice that spills at their 1024 threads (128 VGPRs); the wide kernel
+// (kernels_wide.hip) takes those shapes with `a` in LDS.
+bool stream_m_supported(int fp64, int M) { return fp64 ? M == 8 : M == 16; }
 
 template <typename REAL>
 bool stream_cfg_valid(const Geom &g, const HybridCfg &c) {
@@ -252,24 +254,10 @@ bool stream_cfg_valid(const Geom &g, const HybridCfg &c) {
 template <typename REAL, bool INC>
 hipError_t launch_stream(const Geom &g, const HybridCfg &cfg, const REAL *a, const REAL *b,
                          const REAL *c, REAL *d, REAL *u, hipStream_t st) {
-  if (cfg.V == 2) {
-    if constexpr (sizeof(REAL) == 8) {
-      if (cfg.M == 8) return launch_lds_m<REAL, 8, INC>(g, cfg, a, b, c, d, u, st);
-      if (cfg.M == 16) return launch_lds_m<REAL, 16, INC>(g, cfg, a, b, c, d, u, st);
-    } else {
-      if (cfg.M == 16) return launch_lds_m<REAL, 16, INC>(g, cfg, a, b, c, d, u, st);
-      if (cfg.M == 32) return launch_lds_m<REAL, 32, INC>(g, cfg, a, b, c, d, u, st);
-    }
-    return hipErrorInvalidValue;
-  }
-  if constexpr (sizeof(REAL) == 8) {
-    if (cfg.M == 8) return launch_stream_m<REAL, 8, INC>(g, cfg, a, b, c, d, u, st);
-    if (cfg.M == 16) return launch_stream_m<REAL, 16, INC>(g, cfg, a, b, c, d, u, st);
-  } else {
-    if (cfg.M == 16) return launch_stream_m<REAL, 16, INC>(g, cfg, a, b, c, d, u, st);
-    if (cfg.M == 32) return launch_stream_m<REAL, 32, INC>(g, cfg, a, b, c, d, u, st);
-  }
-  return hipErrorInvalidValue;
+  constexpr int M = sizeof(REAL) == 8 ? 8 : 16;
+  if (cfg.M != M) return hipErrorInvalidValue;
+  if (cfg.V == 2) return launch_lds_m<REAL, M, INC>(g, cfg, a, b, c, d, u, st);
+  return launch_stream_m<REAL, M, INC>(g, cfg, a, b, c, d, u, st);
 }
 
 #define TRID_STREAM_INST(REAL)                                                               \

@@ -190,7 +190,7 @@ EXTERN_C tridStatus_t tridDmtsvStridedBatchMixedF32(const TridParams *ctx, const
   const HybridCfg &forced = forced_config();
   if (forced.M != 0) {
     cfg = forced;
-    if (!mixed32_cfg_valid(g, cfg)) return TRID_STATUS_INVALID_VALUE;
+    if (!mixed32_cfg_valid(g, cfg, refinements)) return TRID_STATUS_INVALID_VALUE;
   } else if (!mixed32_tuned_config(dev, g, refinements, &cfg)) {
     return TRID_STATUS_INVALID_VALUE;
   }

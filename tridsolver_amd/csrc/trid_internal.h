@@ -111,7 +111,9 @@ hipError_t launch_mixed(const Geom &g, const HybridCfg &cfg, int refine, const d
                         const double *b, const double *c, double *d, hipStream_t st);
 
 // ---- FP64 solution of systems stored in FP32 (kernels_mixed32.hip) -------
-bool mixed32_cfg_valid(const Geom &g, const HybridCfg &c);
+// refine: the refinements the shape is to run (more than one needs a
+// general-loop kernel of the shape)
+bool mixed32_cfg_valid(const Geom &g, const HybridCfg &c, int refine);
 bool mixed32_default_cfg(const Geom &g, int refine, HybridCfg *cfg);
 hipError_t launch_mixed32(const Geom &g, const HybridCfg &cfg, int refine, const float *a,
                           const float *b, const float *c, const float *d, double *x,

@@ -39,7 +39,8 @@ namespace {
 enum Kind : int { KIND_SOLVE = 0, KIND_MIXED = 1, KIND_RAGGED = 2, KIND_MIXED32 = 3, KIND_ADJOINT = 4 };
 
 struct Key {
-  int dev, kind, fp64, inc;
+  int dev, kind, fp64;
+  int inc;  // the Inc solve; for KIND_MIXED32: more than one refinement
   int64_t N, stride, n0, n1, st0, st1;
   bool operator<(const Key &o) const {
     return std::tie(dev, kind, fp64, inc, N, stride, n0, n1, st0, st1) <
@@ -192,22 +193,6 @@ std::vector<HybridCfg> candidates(const Geom &g) {
           if (hybrid_cfg_valid<REAL>(g, c)) out.push_back(c);
         }
   }
-  if (g.stride != 1) {  // the streamed variant also runs M = 32 (FP32)
-    for (int W : Ws)
-      for (int G : Gs) {
-        HybridCfg c;
-        c.M = 32;
-        c.P = (int)((g.N + 31) / 32);
-        c.W = W;
-        c.G = G;
-        if ((int64_t)W * (int64_t)sizeof(REAL) < 64 && W < g.n0) continue;
-        if ((int64_t)W > g.n0 && W > 1) continue;
-        for (int V : {1, 2}) {
-          c.V = V;
-          if (hybrid_cfg_valid<REAL>(g, c)) out.push_back(c);
-        }
-      }
-  }
   return out;
 }
 
@@ -342,11 +327,13 @@ tridStatus_t tune_mixed(int dev, const Geom &g, Choice *best) {
   return best->algo ? TRID_STATUS_SUCCESS : TRID_STATUS_EXECUTION_FAILED;
 }
 
-// The FP32-stored solve (tridDmtsvStridedBatchMixedF32) at one refinement:
-// contiguous systems the register kernel over M x workgroup size x G;
-// strided ones the wide kernel (128- and 64-byte rows) over G, and the
-// register kernel at 16 and 8 rows per lane.
-tridStatus_t tune_mixed32(int dev, const Geom &g, Choice *best) {
+// The FP32-stored solve (tridDmtsvStridedBatchMixedF32) at `refine`
+// refinements (1, or 2 for the shapes of more than one): contiguous systems
+// the register kernel and the kernel with a and c in LDS over M x workgroup
+// size x G; strided ones the wide kernel (128- and 64-byte rows) over G, and
+// the register kernel at 16 and 8 rows per lane. Shapes without a kernel for
+// `refine` are skipped (mixed32_cfg_valid).
+tridStatus_t tune_mixed32(int dev, const Geom &g, int refine, Choice *best) {
   (void)dev;
   hipStream_t st = nullptr;
   if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess)
@@ -367,17 +354,18 @@ tridStatus_t tune_mixed32(int dev, const Geom &g, Choice *best) {
   best->ms = INFINITY;
   std::vector<HybridCfg> cands;
   static const int Gs[] = {1, 8, 16, 32};
-  if (g.stride == 1) {
-    for (int M : {8, 16})
-      for (int W : {1, 2, 4, 8, 16})
-        for (int G : {1, 8}) {
-          HybridCfg cfg;
-          cfg.M = M; cfg.P = (int)((g.N + M - 1) / M); cfg.W = W; cfg.G = G; cfg.V = 0;
-          if ((int64_t)W > g.n0 && W > 1) continue;
-          cands.push_back(cfg);
-        }
+  if (g.stride == 1) {  // V = 1: a and c kept in LDS
+    for (int V : {0, 1})
+      for (int M : {8, 16})
+        for (int W : {1, 2, 4, 8, 16})
+          for (int G : {1, 8}) {
+            HybridCfg cfg;
+            cfg.M = M; cfg.P = (int)((g.N + M - 1) / M); cfg.W = W; cfg.G = G; cfg.V = V;
+            if ((int64_t)W > g.n0 && W > 1) continue;
+            cands.push_back(cfg);
+          }
   } else {
-    const int wide[][2] = {{32, 32}, {16, 32}, {32, 16}};  // {M, W}
+    const int wide[][2] = {{32, 32}, {32, 16}, {16, 16}};  // {M, W}
     for (const auto &mw : wide)
       for (int G : Gs) {
         HybridCfg cfg;
@@ -387,20 +375,23 @@ tridStatus_t tune_mixed32(int dev, const Geom &g, Choice *best) {
         cfg.P = (cfg.P + per_wave - 1) / per_wave * per_wave;
         cands.push_back(cfg);
       }
-    for (int M : {8, 16})
-      for (int W : {4, 8, 16}) {
-        HybridCfg cfg;
-        cfg.M = M; cfg.P = (int)((g.N + M - 1) / M); cfg.W = W; cfg.G = 8; cfg.V = 0;
-        cands.push_back(cfg);
-      }
+    for (int V : {0, 1}) {  // V = 1 (a and c in LDS) for more than one refinement
+      if (V == 1 && refine <= 1) continue;
+      for (int M : {8, 16})
+        for (int W : {4, 8, 16}) {
+          HybridCfg cfg;
+          cfg.M = M; cfg.P = (int)((g.N + M - 1) / M); cfg.W = W; cfg.G = 8; cfg.V = V;
+          cands.push_back(cfg);
+        }
+    }
   }
   // median of 3 for every candidate, then median of 9 for the fastest few
   // (the z shapes at XCD grouping 16 and 32 lie within a few percent)
   std::vector<std::pair<float, HybridCfg>> timed;
   for (const HybridCfg &cfg : cands) {
-    if (!mixed32_cfg_valid(g, cfg)) continue;
+    if (!mixed32_cfg_valid(g, cfg, refine)) continue;
     const float ms =
-        time_on(st, [&] { return launch_mixed32(g, cfg, 1, a, b, c, d, x, st); });
+        time_on(st, [&] { return launch_mixed32(g, cfg, refine, a, b, c, d, x, st); });
     if (std::isfinite(ms)) timed.emplace_back(ms, cfg);
   }
   std::sort(timed.begin(), timed.end(),
@@ -411,7 +402,7 @@ tridStatus_t tune_mixed32(int dev, const Geom &g, Choice *best) {
   for (size_t k = 0; k < timed.size() && k < FINALISTS; ++k) {
     const HybridCfg &cfg = timed[k].second;
     const float ms =
-        time_on(st, [&] { return launch_mixed32(g, cfg, 1, a, b, c, d, x, st); }, 9);
+        time_on(st, [&] { return launch_mixed32(g, cfg, refine, a, b, c, d, x, st); }, 9);
     if (ms < best->ms) {
       best->ms = ms;
       best->algo = 10;
@@ -464,22 +455,21 @@ bool mixed_tuned_config(int dev, const Geom &g, HybridCfg *cfg) {
   return mixed_default_cfg(g, cfg);
 }
 
-// Tuned shapes are timed at one refinement; more refinements keep the
-// defaults, which avoid the shapes that spill there.
+// Two tuned shapes per geometry: for at most one refinement (timed at 1) and
+// for more (timed at 2; the key's second flag).
 bool mixed32_tuned_config(int dev, const Geom &g, int refine, HybridCfg *cfg) {
-  if (refine <= 1) {
-    const Key k = key_of(dev, KIND_MIXED32, 0, 0, g);
-    Choice c;
-    if (!lookup(k, &c) && autotune_mode() == 1 && !capturing(current_stream())) {
-      if (tune_mixed32(dev, g, &c) == TRID_STATUS_SUCCESS) store(k, c);
-      else c.algo = 0;
-    } else if (!lookup(k, &c)) {
-      c.algo = 0;
-    }
-    if (c.algo == 10 && mixed32_cfg_valid(g, c.cfg)) {
-      *cfg = c.cfg;
-      return true;
-    }
+  const bool general = refine > 1;
+  const Key k = key_of(dev, KIND_MIXED32, 0, general, g);
+  Choice c;
+  if (!lookup(k, &c) && autotune_mode() == 1 && !capturing(current_stream())) {
+    if (tune_mixed32(dev, g, general ? 2 : 1, &c) == TRID_STATUS_SUCCESS) store(k, c);
+    else c.algo = 0;
+  } else if (!lookup(k, &c)) {
+    c.algo = 0;
+  }
+  if (c.algo == 10 && mixed32_cfg_valid(g, c.cfg, refine)) {
+    *cfg = c.cfg;
+    return true;
   }
   return mixed32_default_cfg(g, refine, cfg);
 }
@@ -707,9 +697,11 @@ EXTERN_C tridStatus_t tridHipAutotuneMixedF32(int ndim, int solvedim, const int 
   int dev = 0;
   if ((s = check_device(&dev)) != TRID_STATUS_SUCCESS) return s;
   if (empty) return TRID_STATUS_SUCCESS;
-  Choice c;
-  if ((s = tune_mixed32(dev, g, &c)) != TRID_STATUS_SUCCESS) return s;
+  Choice c, c2;
+  if ((s = tune_mixed32(dev, g, 1, &c)) != TRID_STATUS_SUCCESS) return s;
+  if ((s = tune_mixed32(dev, g, 2, &c2)) != TRID_STATUS_SUCCESS) return s;
   store(key_of(dev, KIND_MIXED32, 0, 0, g), c);
+  store(key_of(dev, KIND_MIXED32, 0, 1, g), c2);
   if (best_ms) *best_ms = c.ms;
   return TRID_STATUS_SUCCESS;
 }
