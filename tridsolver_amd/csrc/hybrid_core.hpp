@@ -462,6 +462,60 @@ __device__ __forceinline__ void reduced_solve(const Lane &L, Ring &ring, TR al_e
   ym = ymv[0];
 }
 
+// Phases (i)-(iii) with the chunk's `a` kept in LDS: al_s[r * S] holds a of
+// the lane's row r (not read for the system's first row: first_chunk). The
+// forward sweep overwrites it with al, the backward sweep with ap; rc and rd
+// become gp and dp. y and ym as from reduced_solve. The arithmetic is
+// solve_chunk's; only where a lives differs. One row's LDS traffic at a time
+// (scheduling barriers), so the compiler does not hoist every LDS read.
+template <typename TR, int M, int S>
+__device__ __forceinline__ void sweep_lds_a(const Lane &L, Ring &ring, TR *al_s,
+                                            bool first_chunk, const TR (&rb)[M], TR (&rc)[M],
+                                            TR (&rd)[M], TR &y, TR &ym) {
+  // (i) forward sweep with left spike; al in the tile, ga in rc, de in rd
+  TR al_prev = 0;
+#pragma unroll
+  for (int r = 0; r < M; ++r) {
+    const TR ar = (r > 0 || !first_chunk) ? al_s[r * S] : TR(0);  // a[0] is not used
+    TR al;
+    if (r == 0) {
+      const TR inv = TR(1) / rb[0];
+      al = ar * inv;
+      rc[0] = rc[0] * inv;
+      rd[0] = rd[0] * inv;
+    } else {
+      const TR inv = TR(1) / (rb[r] - ar * rc[r - 1]);
+      al = -ar * al_prev * inv;
+      rc[r] = rc[r] * inv;
+      rd[r] = (rd[r] - ar * rd[r - 1]) * inv;
+    }
+    al_s[r * S] = al;
+    al_prev = al;
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  const TR al_e = al_prev, ga_e = rc[M - 1], de_e = rd[M - 1];
+  // (ii) backward sweep; ap replaces al in the tile, gp and dp replace ga, de
+  TR ap_next = M >= 2 ? al_s[(M - 2) * S] : TR(0);
+#pragma unroll
+  for (int r = M - 3; r >= 0; --r) {
+    const TR gr = rc[r];
+    rd[r] = rd[r] - gr * rd[r + 1];
+    const TR ap = al_s[r * S] - gr * ap_next;
+    al_s[r * S] = ap;
+    ap_next = ap;
+    rc[r] = -gr * rc[r + 1];
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  TR f_d, f_a, f_g;
+  if (M >= 2) {
+    f_d = rd[0]; f_a = ap_next; f_g = rc[0];
+  } else {
+    f_d = TR(0); f_a = TR(0); f_g = TR(-1);
+  }
+  // (iii) reduced system across the P chunks
+  reduced_solve<TR>(L, ring, al_e, ga_e, de_e, f_d, f_a, f_g, y, ym);
+}
+
 // (iv) x_r = dp_r - ap_r ym - gp_r y; x_{M-1} = y.
 template <typename TR, int M>
 __device__ __forceinline__ void substitute(const TR (&ra)[M], const TR (&rc)[M],

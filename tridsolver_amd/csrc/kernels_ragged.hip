@@ -153,17 +153,37 @@ struct RgMeta {
   int64_t base;
 };
 
-__device__ __forceinline__ RgMeta rg_meta(const RaggedPlan *plan, const int *__restrict__ list,
+// The plan's class prefixes, copied into LDS once per workgroup at the start
+// of a solve launch (rg_prefix has written them before it): item decoding
+// then reads them from LDS, with no chain of dependent global loads.
+struct RgPlanView {
+  int64_t sys_prefix[RG_CLASSES + 1];
+  int64_t item_prefix[RG_CLASSES + 1];
+};
+
+// Call with every thread before the launch's first barrier.
+__device__ __forceinline__ void rg_view_load(RgPlanView &v, const RaggedPlan *plan, int t) {
+  if (t <= RG_CLASSES) {
+    v.sys_prefix[t] = plan->sys_prefix[t];
+    v.item_prefix[t] = plan->item_prefix[t];
+  }
+}
+
+__device__ __forceinline__ RgMeta rg_meta(const RgPlanView &v, const int *__restrict__ list,
                                           const int64_t *__restrict__ off, int item, int t,
                                           int Tmax) {
   RgMeta m;
+  // scheduling slot k: the last one whose first item is <= item (the
+  // prefixes do not decrease, so that is the count of k >= 1 whose first
+  // item is <= item; independent LDS reads)
   int k = 0;
-  while (k + 1 < RG_CLASSES && plan->item_prefix[k + 1] <= item) ++k;
+#pragma unroll
+  for (int j = 1; j < RG_CLASSES; ++j) k += v.item_prefix[j] <= item ? 1 : 0;
   m.cls = RG_CLASSES - 1 - k;
   const int P = 1 << m.cls, W = Tmax >> m.cls;
-  const int64_t first = plan->sys_prefix[m.cls] + (int64_t)(item - plan->item_prefix[k]) * W;
+  const int64_t first = v.sys_prefix[m.cls] + (int64_t)(item - v.item_prefix[k]) * W;
   const int64_t slot = first + t / P;
-  m.active = slot < plan->sys_prefix[m.cls + 1];
+  m.active = slot < v.sys_prefix[m.cls + 1];
   const int64_t sys = m.active ? list[slot] : 0;
   m.base = m.active ? off[sys] : 0;
   m.N = m.active ? (int)(off[sys + 1] - off[sys]) : 0;
@@ -201,6 +221,32 @@ struct RgBounds {
 // else the rest (T = Tshort, two workgroups per CU, so that one loads while
 // the other computes). A single launch plans with Tshort = Tmax: no item is
 // long, and the LONG = false launch at T = Tmax takes them all.
+// One work item of rg_solve: W systems of one class, P lanes each, in T
+// threads. `fetched()` runs once the loads are issued and a barrier has
+// passed (the caller decodes its next item there).
+template <typename REAL, int M, bool INC, int T, typename OnFetched>
+__device__ __forceinline__ void rg_item(const RgMeta &cur, unsigned char *smem_raw,
+                                        const REAL *__restrict__ a, const REAL *__restrict__ b,
+                                        const REAL *__restrict__ c, REAL *__restrict__ d,
+                                        REAL *__restrict__ u, OnFetched fetched) {
+  const int t = threadIdx.x;
+  const hyb::Lane L = rg_lane(cur, t, T);
+  REAL *tile = reinterpret_cast<REAL *>(smem_raw) + (size_t)L.w * L.P * (M + 1);
+  hyb::Ring ring{smem_raw + sizeof(REAL) * (size_t)T * (M + 1), (int)(3 * sizeof(REAL) * T), 0};
+  REAL ra[M], rb[M], rc[M], rd[M];
+  hyb::fetch_abcd<M, true>(L, a, b, c, d, ra, rb, rc, rd);
+  hyb::to_chunk<M, true>(L, ra, tile);  // has barriers
+  fetched();
+  hyb::to_chunk<M, true>(L, rb, tile);
+  hyb::to_chunk<M, true>(L, rc, tile);
+  hyb::to_chunk<M, true>(L, rd, tile);
+  REAL x[M];
+  hyb::solve_chunk<REAL, M, true>(L, ring, ra, rb, rc, rd, x);
+  hyb::to_fetch<M, true>(L, x, tile);
+  if (INC) hyb::store_add<M, true>(L, u, x);
+  else hyb::store<M, true>(L, d, x);
+}
+
 template <typename REAL, int M, bool INC, int T, bool LONG>
 __global__ __launch_bounds__(T) void rg_solve(
     const REAL *__restrict__ a, const REAL *__restrict__ b, const REAL *__restrict__ c,
@@ -209,6 +255,8 @@ __global__ __launch_bounds__(T) void rg_solve(
   constexpr int Tmax = T;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ int s_next;
+  __shared__ RgPlanView s_view;
+  rg_view_load(s_view, plan, threadIdx.x);
   const int t = threadIdx.x;
   int *counter = LONG ? &plan->work : &plan->work_short;
   const int lo = LONG ? 0 : plan->long_items;
@@ -216,29 +264,200 @@ __global__ __launch_bounds__(T) void rg_solve(
   if (t == 0) s_next = lo + atomicAdd(counter, 1);
   __syncthreads();
   int item = s_next;
-  RgMeta cur = item < total ? rg_meta(plan, list, off, item, t, Tmax) : RgMeta{};
+  RgMeta cur = item < total ? rg_meta(s_view, list, off, item, t, Tmax) : RgMeta{};
   for (;;) {
     __syncthreads();  // every thread has read s_next
     if (item >= total) break;
     // claim the following item now; its id is read after the next barrier
     if (t == 0) s_next = lo + atomicAdd(counter, 1);
-    const hyb::Lane L = rg_lane(cur, t, Tmax);
-    REAL *tile = reinterpret_cast<REAL *>(smem_raw) + (size_t)L.w * L.P * (M + 1);
-    hyb::Ring ring{smem_raw + sizeof(REAL) * (size_t)Tmax * (M + 1),
-                   (int)(3 * sizeof(REAL) * Tmax), 0};
-    REAL ra[M], rb[M], rc[M], rd[M];
-    hyb::fetch_abcd<M, true>(L, a, b, c, d, ra, rb, rc, rd);
-    hyb::to_chunk<M, true>(L, ra, tile);  // has barriers: s_next is visible below
-    const int nxt = s_next;
-    const RgMeta next = nxt < total ? rg_meta(plan, list, off, nxt, t, Tmax) : RgMeta{};
-    hyb::to_chunk<M, true>(L, rb, tile);
-    hyb::to_chunk<M, true>(L, rc, tile);
-    hyb::to_chunk<M, true>(L, rd, tile);
-    REAL x[M];
-    hyb::solve_chunk<REAL, M, true>(L, ring, ra, rb, rc, rd, x);
-    hyb::to_fetch<M, true>(L, x, tile);
-    if (INC) hyb::store_add<M, true>(L, u, x);
-    else hyb::store<M, true>(L, d, x);
+    int nxt = total;
+    RgMeta next{};
+    rg_item<REAL, M, INC, Tmax>(cur, smem_raw, a, b, c, d, u, [&] {
+      nxt = s_next;
+      if (nxt < total) next = rg_meta(s_view, list, off, nxt, t, Tmax);
+    });
+    item = nxt;
+    cur = next;
+  }
+}
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+// FP64 systems needing more than 256 lanes at 16 rows per lane (4097-8192
+// rows) at 32 rows per lane in 256 threads. b, c and d go to chunk order
+// through the LDS tile as in rg_item; `a` then goes from HBM into the same
+// tile by global->LDS loads, in chunk order, and stays there through the
+// sweeps (hyb::sweep_lds_a), so the lane holds 32 rows of b, c, d only (256
+// VGPRs: two 256-thread workgroups per CU). One wave instruction of those
+// loads moves 256 bytes, rows 32k .. 32k+31, to tile slots 33k .. 33k+31:
+// with one pad slot per 32 rows that is the skewed layout (hyb::skew<32>)
+// the transposes use, so lane p reads its rows at slots 33p + r. The items
+// are single systems (plan class 9 at Tmax 512).
+constexpr int RGL_M = 32, RGL_T = 256, RGL_PLAN_T = 512;
+
+// LDS of rg_solve_split: the long items' tile and ring (the rest need less).
+constexpr size_t rg_split_lds_bytes() {
+  return sizeof(double) * (size_t)RGL_T * (RGL_M + 1) + hyb::ring_bytes<double>(RGL_T, 3);
+}
+
+// v of lane 0, which the compiler then holds in scalar registers (a value
+// every lane of the wave has).
+__device__ __forceinline__ int64_t wave_uniform(int64_t v) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)v >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// `fetched()` as for rg_item.
+template <bool INC, typename OnFetched>
+__device__ __forceinline__ void rg_long_item(const RgMeta &cur, unsigned char *smem_raw,
+                                             const double *__restrict__ a,
+                                             const double *__restrict__ b,
+                                             const double *__restrict__ c,
+                                             double *__restrict__ d, double *__restrict__ u,
+                                             OnFetched fetched) {
+  constexpr int M = RGL_M, T = RGL_T;
+  const int t = threadIdx.x;
+  // the item's system (listed items are active): the same for every lane,
+  // so its base and length sit in scalar registers and rows are 32-bit
+  // offsets from it
+  const int64_t base = wave_uniform(cur.base);
+  const int N = __builtin_amdgcn_readfirstlane(cur.N);
+  hyb::Lane L;
+  L.P = T; L.W = 1; L.T = T; L.t = t; L.p = t; L.w = 0; L.pstep = 1; L.stride = 1;
+  L.active = true; L.base = base; L.N = N; L.i0 = 0; L.i1 = 0;
+  double *tile = reinterpret_cast<double *>(smem_raw);
+  hyb::Ring ring{smem_raw + sizeof(double) * (size_t)T * (M + 1), (int)(3 * sizeof(double) * T),
+                 0};
+  const double *sb = b + base, *sc = c + base, *sd = d + base;
+  // b, c, d in fetch order (lane t: rows t + T*r), from rows clamped into
+  // the system so that every load is unconditional; then identity rows past
+  // N and c of row N-1 set to their values
+  double rb[M], rc[M], rd[M];
+#pragma unroll
+  for (int r = 0; r < M; ++r) {
+    const int row = t + T * r;
+    const int rr = row < N ? row : N - 1;
+    rb[r] = sb[rr];
+    rc[r] = sc[rr];
+    rd[r] = sd[rr];
+  }
+#pragma unroll
+  for (int r = 0; r < M; ++r) {
+    const int row = t + T * r;
+    rb[r] = row < N ? rb[r] : 1.0;
+    rc[r] = row < N - 1 ? rc[r] : 0.0;
+    rd[r] = row < N ? rd[r] : 0.0;
+  }
+  // transposes through the tile, hyb::to_chunk / to_fetch at P = T = 256,
+  // M = 32: fetch-order row t + 256r sits at slot skew<32>(t + 256r) =
+  // (t + t/32) + 264r, so each lane addresses all its slots from one base
+  // with constant offsets (no per-row address registers)
+  const int fslot = t + (t >> 5), cslot = t * (M + 1);
+  auto to_chunk = [&](double (&v)[M]) {
+#pragma unroll
+    for (int r = 0; r < M; ++r) tile[fslot + (T + T / M) * r] = v[r];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < M; ++r) v[r] = tile[cslot + r];
+    __syncthreads();
+  };
+  to_chunk(rb);
+  fetched();
+  to_chunk(rc);
+  to_chunk(rd);  // ends with a barrier: the tile is free
+  {
+    // wave v moves the row groups k = v, v + T/64, ...; lane l the dword l
+    // of the group's 256 bytes (row 32k + l/2). Row 0's a is not used and
+    // rows >= N are masked off. (The thread index passes through an empty
+    // asm: the loop's per-lane values are formed here for each item rather
+    // than hoisted out of the caller's item loop and held across it.)
+    int tt = t;
+    asm volatile("" : "+v"(tt));
+    const int lane = tt & 63;
+    const int rl = lane >> 1;
+    const char *src = reinterpret_cast<const char *>(a + base) + 4 * lane;
+    for (int k = tt >> 6; k * M < N; k += T / 64) {
+      const int row = k * M + rl;
+      if (row >= 1 && row < N)
+        __builtin_amdgcn_global_load_lds(src + k * M * (int)sizeof(double),
+                                         (lds_void *)(tile + k * (M + 1)), 4, 0, 0);
+    }
+  }
+  const int row0 = t * M;
+  double *al_s = tile + t * (M + 1);  // this lane's rows, chunk order
+  // a of the identity rows past N is 0: the lane clears its own slots, which
+  // no load writes
+  if (row0 + M > N) {
+#pragma unroll
+    for (int r = 0; r < M; ++r)
+      if (row0 + r >= N) al_s[r] = 0.0;
+  }
+  // the tile is written by other waves' loads: wait for this wave's, then
+  // for every wave
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  double y, ym;
+  hyb::sweep_lds_a<double, M, 1>(L, ring, al_s, row0 == 0, rb, rc, rd, y, ym);
+  // (iv) substitution into rd; each lane reads only its own slots, which
+  // to_fetch then overwrites first
+#pragma unroll
+  for (int r = 0; r < M - 1; ++r) rd[r] = rd[r] - al_s[r] * ym - rc[r] * y;
+  rd[M - 1] = y;
+#pragma unroll
+  for (int r = 0; r < M; ++r) tile[cslot + r] = rd[r];
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < M; ++r) rd[r] = tile[fslot + (T + T / M) * r];
+  __syncthreads();
+  double *dst = (INC ? u : d) + base;
+#pragma unroll
+  for (int r = 0; r < M; ++r) {
+    const int row = t + T * r;
+    if (row < N) {
+      if (INC) dst[row] += rd[r];
+      else dst[row] = rd[r];
+    }
+  }
+}
+
+// The three-launch split's solve (FP64, plan at 16 rows per lane with Tmax
+// 512 and Tshort 256): one persistent launch of 256-thread workgroups, two
+// per CU, over every item. The long items [0, long_items) come first in the
+// queue and take rg_long_item at 32 rows per lane; the rest take rg_item at
+// 16 rows per lane. One launch, so a workgroup that runs out of long items
+// goes straight on to the short ones.
+template <bool INC>
+__global__ __launch_bounds__(RGL_T, 2) void rg_solve_split(
+    const double *__restrict__ a, const double *__restrict__ b, const double *__restrict__ c,
+    double *__restrict__ d, double *__restrict__ u, const int64_t *__restrict__ off,
+    const int *__restrict__ list, RaggedPlan *plan) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ int s_next;
+  __shared__ RgPlanView s_view;
+  rg_view_load(s_view, plan, threadIdx.x);
+  const int t = threadIdx.x;
+  const int total = plan->total_items, nlong = plan->long_items;
+  auto decode = [&](int i) {
+    return i < nlong ? rg_meta(s_view, list, off, i, 0, RGL_PLAN_T)
+                     : rg_meta(s_view, list, off, i, t, RGL_T);
+  };
+  if (t == 0) s_next = atomicAdd(&plan->work, 1);
+  __syncthreads();
+  int item = s_next;
+  RgMeta cur = item < total ? decode(item) : RgMeta{};
+  for (;;) {
+    __syncthreads();  // every thread has read s_next
+    if (item >= total) break;
+    if (t == 0) s_next = atomicAdd(&plan->work, 1);
+    int nxt = total;
+    RgMeta next{};
+    auto fetched = [&] {
+      nxt = s_next;
+      if (nxt < total) next = decode(nxt);
+    };
+    if (item < nlong) rg_long_item<INC>(cur, smem_raw, a, b, c, d, u, fetched);
+    else rg_item<double, 16, INC, RGL_T>(cur, smem_raw, a, b, c, d, u, fetched);
     item = nxt;
     cur = next;
   }
@@ -284,8 +503,21 @@ hipError_t launch_m(const REAL *a, const REAL *b, const REAL *c, REAL *d, REAL *
                        off, list, plan);
     return hipGetLastError();
   }
+  if (launches == 3 && !(sizeof(REAL) == 8 && M == 16)) return hipErrorInvalidValue;
   launch_plan(off, S, M, Tmax, Tshort, ws, st);
-  {  // the long systems first (an empty range exits at once)
+  if constexpr (sizeof(REAL) == 8 && M == 16) {
+    if (launches == 3) {  // one launch: the long systems at 32 rows per lane, then the rest
+      auto kern = rg_solve_split<INC>;
+      static LdsGrant lds_granted;
+      hipError_t e = ensure_lds(reinterpret_cast<const void *>(kern), rg_split_lds_bytes(),
+                                lds_granted);
+      if (e != hipSuccess) return e;
+      hipLaunchKernelGGL(kern, dim3((unsigned)(2 * cus * per)), dim3(RGL_T),
+                         rg_split_lds_bytes(), st, a, b, c, d, u, off, list, plan);
+      return hipGetLastError();
+    }
+  }
+  if (launches == 2) {  // the long systems first (an empty range exits at once)
     const size_t lds = sizeof(REAL) * (size_t)Tmax * (M + 1) + hyb::ring_bytes<REAL>(Tmax, 3);
     auto kern = rg_solve<REAL, M, INC, Tmax, true>;
     static LdsGrant lds_granted;
@@ -317,12 +549,14 @@ __global__ __launch_bounds__(RG_MIXED_THREADS) void rg_solve_mixed(
   constexpr int Tmax = RG_MIXED_THREADS;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ int s_next;
+  __shared__ RgPlanView s_view;
+  rg_view_load(s_view, plan, threadIdx.x);
   const int t = threadIdx.x;
   const int total = plan->total_items;
   if (t == 0) s_next = atomicAdd(&plan->work, 1);
   __syncthreads();
   int item = s_next;
-  RgMeta cur = item < total ? rg_meta(plan, list, off, item, t, Tmax) : RgMeta{};
+  RgMeta cur = item < total ? rg_meta(s_view, list, off, item, t, Tmax) : RgMeta{};
   for (;;) {
     __syncthreads();  // every thread has read s_next
     if (item >= total) break;
@@ -335,7 +569,7 @@ __global__ __launch_bounds__(RG_MIXED_THREADS) void rg_solve_mixed(
     hyb::fetch_abcd<M, true>(L, a, b, c, d, A, B, Cc, D);
     hyb::to_chunk<M, true>(L, A, tile);
     const int nxt = s_next;
-    const RgMeta next = nxt < total ? rg_meta(plan, list, off, nxt, t, Tmax) : RgMeta{};
+    const RgMeta next = nxt < total ? rg_meta(s_view, list, off, nxt, t, Tmax) : RgMeta{};
     hyb::to_chunk<M, true>(L, B, tile);
     hyb::to_chunk<M, true>(L, Cc, tile);
     hyb::to_chunk<M, true>(L, D, tile);
@@ -365,12 +599,14 @@ __global__ __launch_bounds__(RG_MIXED_THREADS) void rg_solve_mixed32(
   constexpr int M = RG_M32_M, Tmax = RG_MIXED_THREADS;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ int s_next;
+  __shared__ RgPlanView s_view;
+  rg_view_load(s_view, plan, threadIdx.x);
   const int t = threadIdx.x;
   const int total = plan->total_items;
   if (t == 0) s_next = atomicAdd(&plan->work, 1);
   __syncthreads();
   int item = s_next;
-  RgMeta cur = item < total ? rg_meta(plan, list, off, item, t, Tmax) : RgMeta{};
+  RgMeta cur = item < total ? rg_meta(s_view, list, off, item, t, Tmax) : RgMeta{};
   for (;;) {
     __syncthreads();  // every thread has read s_next
     if (item >= total) break;
@@ -384,7 +620,7 @@ __global__ __launch_bounds__(RG_MIXED_THREADS) void rg_solve_mixed32(
     hyb::fetch_abcd<M, true>(L, a, b, c, d, A, B, Cc, D);
     hyb::to_chunk<M, true>(L, B, ftile);  // has barriers: s_next is visible below
     const int nxt = s_next;
-    const RgMeta next = nxt < total ? rg_meta(plan, list, off, nxt, t, Tmax) : RgMeta{};
+    const RgMeta next = nxt < total ? rg_meta(s_view, list, off, nxt, t, Tmax) : RgMeta{};
     hyb::to_chunk<M, true>(L, D, ftile);
     double X[M];
     if constexpr (AC_LDS) {
@@ -482,6 +718,8 @@ __global__ __launch_bounds__(T, 2) void rg_solve_mixed_lds(
   constexpr int M = RG_MIXED_LDS_M;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ int s_next;
+  __shared__ RgPlanView s_view;
+  rg_view_load(s_view, plan, threadIdx.x);
   const int t = threadIdx.x;
   int *counter = LONG ? &plan->work : &plan->work_short;
   const int lo = LONG ? 0 : plan->long_items;
@@ -489,7 +727,7 @@ __global__ __launch_bounds__(T, 2) void rg_solve_mixed_lds(
   if (t == 0) s_next = lo + atomicAdd(counter, 1);
   __syncthreads();
   int item = s_next;
-  RgMeta cur = item < total ? rg_meta(plan, list, off, item, t, T) : RgMeta{};
+  RgMeta cur = item < total ? rg_meta(s_view, list, off, item, t, T) : RgMeta{};
   double *tiles = reinterpret_cast<double *>(smem_raw);
   hyb::Ring ring{smem_raw + 2 * sizeof(double) * (size_t)T * (M + 1),
                  (int)(2 * sizeof(double) * T), 0};
@@ -506,7 +744,7 @@ __global__ __launch_bounds__(T, 2) void rg_solve_mixed_lds(
     // at most T lanes per system: log2(T) PCR steps
     mixed_contig_lds<M, T == 512 ? 9 : 8>(L, ring, a, b, c, d, tA, tC, refine, [&] {
       nxt = s_next;
-      if (nxt < total) next = rg_meta(plan, list, off, nxt, t, T);
+      if (nxt < total) next = rg_meta(s_view, list, off, nxt, t, T);
     });
     item = nxt;
     cur = next;
@@ -594,7 +832,7 @@ hipError_t launch_ragged(int M, const REAL *a, const REAL *b, const REAL *c, REA
                          const int64_t *off, int64_t S, int per_cu, int launches, void *ws,
                          hipStream_t st, int dev) {
   static_assert(sizeof(RaggedPlan) <= RAGGED_PLAN_HEADER_BYTES, "plan header");
-  if (launches != 1 && launches != 2) return hipErrorInvalidValue;
+  if (launches < 1 || launches > 3) return hipErrorInvalidValue;
   switch (M) {
     case 4: return launch_m<REAL, 4, INC>(a, b, c, d, u, off, S, per_cu, launches, ws, st, dev);
     case 8: return launch_m<REAL, 8, INC>(a, b, c, d, u, off, S, per_cu, launches, ws, st, dev);

@@ -141,49 +141,9 @@ __global__ __launch_bounds__((WideBounds<REAL, M>::threads)) void hybrid_wide_ke
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
 
-  // (i) forward sweep with left spike; al in the tile, ga in rc, de in rd
-  REAL al_prev = 0;
-#pragma unroll
-  for (int r = 0; r < M; ++r) {
-    const REAL ar = (r > 0 || row0 > 0) ? al_s[r * W] : REAL(0);  // a[0] is not used
-    REAL al;
-    if (r == 0) {
-      const REAL inv = REAL(1) / rb[0];
-      al = ar * inv;
-      rc[0] = rc[0] * inv;
-      rd[0] = rd[0] * inv;
-    } else {
-      const REAL inv = REAL(1) / (rb[r] - ar * rc[r - 1]);
-      al = -ar * al_prev * inv;
-      rc[r] = rc[r] * inv;
-      rd[r] = (rd[r] - ar * rd[r - 1]) * inv;
-    }
-    al_s[r * W] = al;
-    al_prev = al;
-    __builtin_amdgcn_sched_barrier(0);  // one row's LDS traffic at a time
-  }
-  const REAL al_e = al_prev, ga_e = rc[M - 1], de_e = rd[M - 1];
-  // (ii) backward sweep; ap replaces al in the tile, gp and dp replace ga, de
-  REAL ap_next = M >= 2 ? al_s[(M - 2) * W] : REAL(0);
-#pragma unroll
-  for (int r = M - 3; r >= 0; --r) {
-    const REAL gr = rc[r];
-    rd[r] = rd[r] - gr * rd[r + 1];
-    const REAL ap = al_s[r * W] - gr * ap_next;
-    al_s[r * W] = ap;
-    ap_next = ap;
-    rc[r] = -gr * rc[r + 1];
-    __builtin_amdgcn_sched_barrier(0);
-  }
-  REAL f_d, f_a, f_g;
-  if (M >= 2) {
-    f_d = rd[0]; f_a = ap_next; f_g = rc[0];
-  } else {
-    f_d = REAL(0); f_a = REAL(0); f_g = REAL(-1);
-  }
-  // (iii) reduced system across the P chunks
+  // (i)-(iii) with al, then ap, in the tile
   REAL y, ym;
-  hyb::reduced_solve<REAL>(L, ring, al_e, ga_e, de_e, f_d, f_a, f_g, y, ym);
+  hyb::sweep_lds_a<REAL, M, W>(L, ring, al_s, row0 == 0, rb, rc, rd, y, ym);
   if (!L.active) return;
   // (iv) substitution. The store offsets are recomputed here (the empty asm
   // hides that they equal the load offsets, which would otherwise be kept

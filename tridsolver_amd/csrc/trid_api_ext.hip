@@ -30,6 +30,8 @@ tridStatus_t ragged(const TridParams *ctx, const REAL *a, const REAL *b, const R
   if (M == 0) ragged_tuned_config<REAL>(dev, max_len, &M, &per_cu, &launches);
   const int supported = ragged_max_len(sizeof(REAL) == 8, M);
   if (supported == 0 || max_len > supported) return TRID_STATUS_INVALID_VALUE;
+  // the three-launch split is built for FP64 at 16 rows per lane
+  if (launches == 3 && !(sizeof(REAL) == 8 && M == 16)) return TRID_STATUS_INVALID_VALUE;
   void *ws = nullptr;
   if ((s = get_workspace(dev, st, ragged_workspace_bytes(nsys), &ws)) != TRID_STATUS_SUCCESS)
     return s;
@@ -66,8 +68,10 @@ tridStatus_t ragged_mixed(const TridParams *ctx, const double *a, const double *
   if ((s = get_workspace(dev, st, ragged_workspace_bytes(nsys), &ws)) != TRID_STATUS_SUCCESS)
     return s;
   HybridCfg rec;
-  // M = 4 and 8 always run in one launch
+  // M = 4 and 8 always run in one launch; the mixed solve has no
+  // three-launch split
   const int launches = M == 16 ? tl_ragged_launches : 1;
+  if (launches == 3) return TRID_STATUS_INVALID_VALUE;
   rec.M = M; rec.P = 0; rec.W = launches; rec.G = tl_ragged_per_cu;
   record_config(9, &rec);
   s = finish(ctx, st, launch_ragged_mixed(M, a, b, c, d, offsets, nsys, tl_ragged_per_cu,
@@ -257,8 +261,8 @@ EXTERN_C tridStatus_t tridHipForceRaggedConfig(int M, int per_cu, int launches) 
     tl_ragged_launches = 1;
     return TRID_STATUS_SUCCESS;
   }
-  if ((M != 4 && M != 8 && M != 16) || per_cu < 1 || per_cu > 8 ||
-      (launches != 1 && launches != 2))
+  if ((M != 4 && M != 8 && M != 16) || per_cu < 1 || per_cu > 8 || launches < 1 ||
+      launches > 3)
     return TRID_STATUS_INVALID_VALUE;
   tl_ragged_m = M;
   tl_ragged_per_cu = per_cu;

@@ -565,7 +565,7 @@ void ragged_tuned_config(int dev, int64_t max_len, int *M, int *per_cu, int *lau
   if (lookup(k, &c) && c.algo == 5 && ragged_max_len(sizeof(REAL) == 8, c.cfg.M) >= max_len) {
     *M = c.cfg.M;
     *per_cu = c.cfg.G;
-    *launches = c.cfg.V == 2 ? 2 : 1;
+    *launches = (c.cfg.V >= 1 && c.cfg.V <= 3) ? c.cfg.V : 1;
     return;
   }
   *launches = 1;
@@ -601,25 +601,30 @@ tridStatus_t tune_ragged(int dev, const int64_t *offsets, int64_t nsys, int64_t 
   Choice best;
   best.algo = 0;
   best.ms = INFINITY;
-  // One persistent solve launch. Splitting off the short systems into a
-  // second launch of half-size workgroups measured 2 % (FP64) and 3 % (FP32)
-  // faster on 40,000 systems of 8-8192 rows; it stays a forced option
-  // (tridHipForceRaggedConfig) so that the batch is one launch by default.
+  // One persistent solve launch, and for FP64 batches with systems over 4096
+  // rows the three-launch split (those systems at 32 rows per lane, two
+  // workgroups per CU, then the rest in two launches by length). The
+  // two-launch split (tridHipForceRaggedConfig) measured within 2-3 % of one
+  // launch on 40,000 systems of 8-8192 rows and is not timed here.
+  struct Cand { int M, per_cu, launches; };
+  std::vector<Cand> cands;
   for (int M : {4, 8, 16})
-    for (int per_cu : {1, 2}) {
-      if (ragged_max_len(sizeof(REAL) == 8, M) < max_len) continue;
-      const float ms = time_on(st, [&] {
-        return launch_ragged<REAL, false>(M, a, b, c, d, nullptr, offsets, nsys, per_cu, 1, ws,
-                                          st, dev);
-      });
-      if (ms < best.ms) {
-        best.ms = ms;
-        best.algo = 5;
-        best.cfg.M = M;
-        best.cfg.G = per_cu;
-        best.cfg.V = 1;
-      }
+    for (int per_cu : {1, 2}) cands.push_back({M, per_cu, 1});
+  if (sizeof(REAL) == 8 && max_len > 4096) cands.push_back({16, 1, 3});
+  for (const Cand &cd : cands) {
+    if (ragged_max_len(sizeof(REAL) == 8, cd.M) < max_len) continue;
+    const float ms = time_on(st, [&] {
+      return launch_ragged<REAL, false>(cd.M, a, b, c, d, nullptr, offsets, nsys, cd.per_cu,
+                                        cd.launches, ws, st, dev);
+    });
+    if (ms < best.ms) {
+      best.ms = ms;
+      best.algo = 5;
+      best.cfg.M = cd.M;
+      best.cfg.G = cd.per_cu;
+      best.cfg.V = cd.launches;
     }
+  }
   (void)hipStreamSynchronize(st);
   (void)hipStreamDestroy(st);
   if (!best.algo) return TRID_STATUS_EXECUTION_FAILED;
