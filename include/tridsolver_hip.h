@@ -234,9 +234,9 @@ EXTERN_C tridStatus_t tridDmtsvRaggedBatchNoAlloc(const TridParams *ctx, const d
                                                   int64_t max_len);
 /* Forces the ragged kernel's rows per lane (4, 8 or 16), workgroups per CU
  * and solve launches (1; 2: systems needing more than half a workgroup's
- * lanes, then the rest in half-size workgroups, two per CU; 3, FP64 at M = 16
- * only: as 2, with the long systems (over 4096 rows) at 32 rows per lane in
- * 256-thread workgroups, two per CU) for this thread's calls; M = 0 restores
+ * lanes, then the rest in half-size workgroups, two per CU; 3, at M = 16
+ * only: one launch of 256-thread workgroups taking the systems over 4096 rows
+ * at 32 rows per lane, then the rest) for this thread's calls; M = 0 restores
  * the tuned choice. The mixed ragged solve takes the same settings (its M = 4
  * and 8 always run in one launch; 3 is refused). */
 EXTERN_C tridStatus_t tridHipForceRaggedConfig(int M, int per_cu, int launches);

@@ -543,21 +543,23 @@ def test_ragged_vs_oracle(dtype, M, launches):
             ctx.__exit__(None, None, None)
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("per_cu", [1, 2])
-def test_ragged_three_launches_fp64(per_cu):
-    """FP64 batch with systems over 4096 rows through the three-launch split:
-    the long systems at 32 rows per lane with `a` moved into the LDS tile by
-    global->LDS loads, then the rest. Lengths at the class edges (4096, 4097,
-    8191, 8192 rows), empty and one-row systems, plain and Inc, against the
-    oracle; the split is refused for FP32 and for the mixed solve."""
+def test_ragged_split_launch(per_cu, dtype):
+    """Batch with systems over 4096 rows through the split launch: the long
+    systems at 32 rows per lane with `a` moved into the LDS tile by
+    global->LDS loads (FP32: two half-wave loads per 64 rows), then the rest.
+    Lengths at the class edges (4096, 4097, 8191, 8192 rows), empty and
+    one-row systems, plain and Inc, against the oracle; the split is refused
+    at other rows per lane and for the mixed solve."""
     rng = np.random.default_rng(33 + per_cu)
     lens = np.round(np.exp(rng.uniform(np.log(1), np.log(8192), 1200))).astype(np.int64)
-    lens[:9] = [0, 1, 4096, 4097, 8191, 8192, 2, 0, 5000]
-    off, a, b, c, d = ragged_case(rng, lens, np.float64)
+    lens[:10] = [0, 1, 4096, 4097, 8191, 8192, 2, 0, 5000, 4160]
+    off, a, b, c, d = ragged_case(rng, lens, dtype)
     with api.forced_ragged(16, per_cu=per_cu, launches=3):
         for inc in (False, True):
             td = dev(d)
-            tu = dev(rng.uniform(-1, 1, len(d))) if inc else None
+            tu = dev(rng.uniform(-1, 1, len(d)).astype(dtype)) if inc else None
             u0 = None if tu is None else tu.cpu().numpy()
             api.ragged_batch(dev(a), dev(b), dev(c), td, dev(off), max_len=8192, u=tu,
                              sync=True)
@@ -565,13 +567,15 @@ def test_ragged_three_launches_fp64(per_cu):
             got = (tu if inc else td).cpu().numpy()
             if inc:
                 assert np.array_equal(td.cpu().numpy(), d)
-            check_ragged(off, a, b, c, d, got, np.float64, u0)
-        t32 = [dev(v.astype(np.float32)) for v in (a, b, c, d)]
+            check_ragged(off, a, b, c, d, got, dtype, u0)
+        if dtype == np.float64:
+            with pytest.raises(Exception):
+                api.ragged_batch_mixed(dev(a), dev(b), dev(c), dev(d), dev(off), max_len=8192,
+                                       sync=True)
+    with api.forced_ragged(8, launches=3):
         with pytest.raises(Exception):
-            api.ragged_batch(*t32, dev(off), max_len=8192, sync=True)
-        with pytest.raises(Exception):
-            api.ragged_batch_mixed(dev(a), dev(b), dev(c), dev(d), dev(off), max_len=8192,
-                                   sync=True)
+            api.ragged_batch(dev(a), dev(b), dev(c), dev(d), dev(off), max_len=4096,
+                             sync=True)
 
 
 @pytest.mark.parametrize("max_len", [2048, 8192])

@@ -316,9 +316,9 @@ def ragged_batch_mixed_f32(a, b, c, d, offsets, x=None, *, max_len, refinements=
 
 class forced_ragged:
     """Context manager forcing the ragged kernel's rows per lane, workgroups
-    per CU and solve launches (1; 2 with the short systems apart; 3, FP64 at
-    M = 16: as 2, with the systems over 4096 rows at 32 rows per lane) for
-    this thread's calls."""
+    per CU and solve launches (1; 2 with the short systems apart; 3, at M = 16:
+    one launch taking the systems over 4096 rows at 32 rows per lane, then the
+    rest) for this thread's calls."""
 
     def __init__(self, M, per_cu=1, launches=1):
         self.cfg = (M, per_cu, launches)

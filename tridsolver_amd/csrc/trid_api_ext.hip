@@ -30,8 +30,8 @@ tridStatus_t ragged(const TridParams *ctx, const REAL *a, const REAL *b, const R
   if (M == 0) ragged_tuned_config<REAL>(dev, max_len, &M, &per_cu, &launches);
   const int supported = ragged_max_len(sizeof(REAL) == 8, M);
   if (supported == 0 || max_len > supported) return TRID_STATUS_INVALID_VALUE;
-  // the three-launch split is built for FP64 at 16 rows per lane
-  if (launches == 3 && !(sizeof(REAL) == 8 && M == 16)) return TRID_STATUS_INVALID_VALUE;
+  // the three-launch split is built for 16 rows per lane
+  if (launches == 3 && M != 16) return TRID_STATUS_INVALID_VALUE;
   void *ws = nullptr;
   if ((s = get_workspace(dev, st, ragged_workspace_bytes(nsys), &ws)) != TRID_STATUS_SUCCESS)
     return s;

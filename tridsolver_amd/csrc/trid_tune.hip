@@ -601,16 +601,16 @@ tridStatus_t tune_ragged(int dev, const int64_t *offsets, int64_t nsys, int64_t 
   Choice best;
   best.algo = 0;
   best.ms = INFINITY;
-  // One persistent solve launch, and for FP64 batches with systems over 4096
-  // rows the three-launch split (those systems at 32 rows per lane, two
-  // workgroups per CU, then the rest in two launches by length). The
-  // two-launch split (tridHipForceRaggedConfig) measured within 2-3 % of one
-  // launch on 40,000 systems of 8-8192 rows and is not timed here.
+  // One persistent solve launch, and for batches with systems over 4096 rows
+  // the split (one launch of 256-thread workgroups: those systems at 32 rows
+  // per lane, then the rest at 16). The two-launch split
+  // (tridHipForceRaggedConfig) measured within 2-3 % of one launch on 40,000
+  // systems of 8-8192 rows and is not timed here.
   struct Cand { int M, per_cu, launches; };
   std::vector<Cand> cands;
   for (int M : {4, 8, 16})
     for (int per_cu : {1, 2}) cands.push_back({M, per_cu, 1});
-  if (sizeof(REAL) == 8 && max_len > 4096) cands.push_back({16, 1, 3});
+  if (max_len > 4096) cands.push_back({16, 1, 3});
   for (const Cand &cd : cands) {
     if (ragged_max_len(sizeof(REAL) == 8, cd.M) < max_len) continue;
     const float ms = time_on(st, [&] {
