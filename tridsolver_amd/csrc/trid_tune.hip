@@ -568,12 +568,15 @@ void ragged_tuned_config(int dev, int64_t max_len, int *M, int *per_cu, int *lau
     *launches = (c.cfg.V >= 1 && c.cfg.V <= 3) ? c.cfg.V : 1;
     return;
   }
-  *launches = 1;
-  // untuned: 16 rows per lane where systems may exceed 4096 rows (the tuned
-  // choice on the 40,000-system batch of lengths up to 8192, FP64 and FP32:
-  // 0.51 against 0.54 ms and 0.333 against 0.348 ms at 8 rows), else 8
-  *per_cu = 1;
+  // untuned: where systems may exceed 4096 rows, 16 rows per lane -- FP64
+  // in the split launch (systems over 4096 rows at 32 rows per lane), FP32
+  // in one launch of two 512-thread workgroups per CU (the tuned choices on
+  // the 40,000-system batch of lengths up to 8192: FP64 0.45 ms against
+  // 0.53 in one launch; FP32 0.27 against 0.30 split and 0.35 at one
+  // workgroup per CU); else 8 rows per lane in one launch
   *M = max_len > 4096 ? 16 : 8;
+  *per_cu = (max_len > 4096 && sizeof(REAL) == 4) ? 2 : 1;
+  *launches = (max_len > 4096 && sizeof(REAL) == 8) ? 3 : 1;
 }
 
 // Tunes the ragged kernel on the caller's offsets with scratch values.
