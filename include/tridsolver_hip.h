@@ -250,7 +250,7 @@ EXTERN_C tridStatus_t tridHipStreamCopy(const void *src, void *dst, size_t bytes
 /* The same copy in a chosen shape: `threads` per workgroup (64..1024, whole
  * waves), `vectors` 16-byte vectors per thread (1, 2 or 4, all loaded before
  * the stores), non-temporal (1) or plain (0) accesses. A sweep of these gives
- * the best copy bandwidth of the box. */
+ * the device's best copy bandwidth. */
 EXTERN_C tridStatus_t tridHipStreamCopyShape(const void *src, void *dst, size_t bytes,
                                              int threads, int vectors, int nontemporal);
 /* The loads and stores of the strided FP64 solve with seg_bytes-wide row

@@ -412,8 +412,8 @@ bool mixed32_cfg_valid(const Geom &g, const HybridCfg &c, int refine) {
   return reg_valid(g, c);
 }
 
-// Default shapes, from sweeps at 512^3 (times on one box; they vary by a few
-// percent between boxes):
+// Default shapes, from sweeps at 512^3 (times vary by a few percent from
+// run to run):
 // - contiguous systems: the kernel with a and c in LDS at 8 rows per lane
 //   (16 past 4096 rows), workgroups of about one wave (W = 64 / P
 //   systems): x 0.61-0.63 ms at one refinement (122 VGPRs, four waves per
