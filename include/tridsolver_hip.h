@@ -50,7 +50,9 @@ EXTERN_C size_t tridHipWorkspaceBytes(int fp64, int inc, int ndim, int solvedim,
  * hybrid, 7 streamed strided hybrid with LDS storage, 8 wide strided hybrid,
  * 9 ragged mixed precision, 10 FP64 solution of FP32-stored systems, 11 the
  * same for a ragged batch, 12 systems longer than one workgroup holds: the
- * hybrid with its reduced system solved by a second batched solve), and
+ * hybrid with its reduced system solved by a second batched solve, 13
+ * contiguous systems of 4097-8192 rows, one per workgroup, with a in LDS),
+ * and
  * for the hybrid its
  * rows per lane M, lanes per system P, systems per workgroup W and XCD
  * grouping G (consecutive system groups dealt to one XCD; 1 = plain order).
@@ -59,11 +61,13 @@ EXTERN_C tridStatus_t tridHipLastConfig(int *algorithm, int *M, int *P, int *W, 
 
 /* Forces the hybrid shape for this thread's subsequent solves (M = 0 clears).
  * variant 0: plain hybrid; 1: streamed strided variant (strided systems,
- * FP64 M = 8/16, FP32 M = 16/32, up to 1024 threads); 2: as 1 with one
- * sweep value per row kept in LDS (arrays spanning < 4 GiB); 3: strided
- * variant with a staged in LDS by global->LDS loads (x-neighbouring strided
- * systems, W * sizeof = 128 or 256 bytes, M = 16/32, W * P a multiple of 64,
- * arrays spanning < 4 GiB). A shape that does not
+ * FP64 M = 8, FP32 M = 16, up to 1024 threads); 2: as 1 with one sweep
+ * value per row kept in LDS (arrays spanning < 4 GiB); 3: strided variant
+ * with a staged in LDS by global->LDS loads (x-neighbouring strided systems,
+ * W * sizeof = 128 or 256 bytes, M = 16/32, W * P a multiple of 64, arrays
+ * spanning < 4 GiB); 4: contiguous systems of 4097-8192 rows, one per
+ * 256-thread workgroup, with a moved into LDS in chunk order by global->LDS
+ * loads (M = 32, P = 256, W = 1). A shape that does not
  * fit a call (P*M < length, too many threads or LDS) makes that call return
  * TRID_STATUS_INVALID_VALUE. */
 EXTERN_C tridStatus_t tridHipForceHybridConfig(int M, int P, int W, int G, int variant);

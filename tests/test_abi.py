@@ -132,7 +132,8 @@ def test_force_hybrid_config_validation():
     assert lib.tridHipForceHybridConfig(-1, 1, 1, 1, 0) == 3
     assert lib.tridHipForceHybridConfig(8, 0, 1, 1, 0) == 3
     assert lib.tridHipForceHybridConfig(8, 1, 1, 0, 0) == 3
-    assert lib.tridHipForceHybridConfig(8, 1, 1, 1, 4) == 3
+    assert lib.tridHipForceHybridConfig(8, 1, 1, 1, 5) == 3  # variants 0-4
+    assert lib.tridHipForceHybridConfig(8, 1, 1, 1, -1) == 3
     assert lib.tridHipForceHybridConfig(0, 0, 0, 0, 0) == 0
 
 

@@ -197,6 +197,45 @@ def test_forced_wide_variant(G, inc, seg, dtype):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("inc", [False, True])
+@pytest.mark.parametrize("G", [1, 8])
+def test_forced_long32_contiguous(G, inc, dtype):
+    """Contiguous systems of 4097-8192 rows, one per 256-thread workgroup at
+    32 rows per lane with `a` moved into the LDS tile by global->LDS loads
+    (variant 4; FP32: two half-wave loads per 64 rows): lengths at both ends
+    of the range and between, Inc; refused for shorter and strided systems."""
+    for dims in ([4097, 3, 2], [5000, 7], [8192, 2, 2], [6001, 5]):
+        with api.forced_hybrid(32, 256, 1, G, 4):
+            cfg = run_case(dims, 0, dtype, inc, (0, 0, 0), seed=dims[0] + G)
+        assert cfg == (13, 32, 256, 1, G)
+    with api.forced_hybrid(32, 256, 1, G, 4):
+        with pytest.raises(Exception):
+            run_case([4096, 3], 0, dtype, inc, (0, 0, 0), seed=1)
+        with pytest.raises(Exception):
+            run_case([3, 5000], 1, dtype, inc, (0, 0, 0), seed=2)
+
+
+def test_forced_long32_padded():
+    """Variant 4 on the interior of a halo-padded allocation (row pitch and
+    plane pitch not multiples of anything): the halo is untouched."""
+    rng = np.random.default_rng(29)
+    dims = [4500, 3, 2]
+    n = int(np.prod(dims))
+    a, b, c, d = H.diag_dominant(rng, n)
+    arrs = [H.pad_halo(t, dims) for t in (a, b, c, d)]
+    off, pads = H.halo_offset(dims), [t + 2 for t in dims]
+    with api.forced_hybrid(32, 256, 1, 8, 4):
+        gd, _ = gpu_solve(*arrs, solvedim=0, dims=dims, pads=pads, offset=off)
+        assert api.last_config()[0] == 13
+    od = arrs[3].copy()
+    oracle.batch(arrs[0], arrs[1], arrs[2], od, ndim=3, solvedim=0, dims=dims, pads=pads,
+                 offset=off)
+    H.assert_norm_close(gd, od, np.float64)
+    mask = H.pad_halo(np.ones(n, dtype=bool), dims)
+    assert np.array_equal(gd[~mask], arrs[3][~mask])
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_forced_wide_padded(dtype):
     """Wide variant on the interior of a halo-padded allocation."""
     rng = np.random.default_rng(23)

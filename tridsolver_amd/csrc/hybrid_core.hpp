@@ -540,5 +540,155 @@ __device__ __forceinline__ void solve_chunk(const Lane &L, Ring &ring, TR (&ra)[
   substitute<TR, M>(ra, rc, rd, y, ym, x);
 }
 
+// ---- one system of 4097-8192 rows in one 256-thread workgroup -----------
+// 32 rows per lane, 256 lanes. b, c and d go to chunk order through the LDS
+// tile; `a` then goes from HBM into the same tile by global->LDS loads, in
+// chunk order, and stays there through the sweeps (sweep_lds_a), so the lane
+// holds 32 rows of b, c, d only (FP64: about 256 VGPRs, two workgroups per
+// CU). The tile is skewed with one pad slot per 32 rows (skew<32>, as the
+// transposes use), so lane p reads its rows at slots 33p + r. A
+// global->LDS wave instruction writes 256 contiguous LDS bytes (lane l:
+// bytes 4l .. 4l+3 from its own source): FP64, rows 32k .. 32k+31 to slots
+// 33k ..; FP32, rows 64j .. 64j+63 by two instructions with complementary
+// lane masks, lanes 0-31 (chunk 2j) to slots 66j .. and lanes 32-63 (chunk
+// 2j+1) from base slot 66j + 1, which puts row 64j + l at slot 66j + 1 + l =
+// 33(2j+1) + (l - 32). base and N are the system's first element and length
+// (4096 < N <= 8192), the same for every lane: callers pass them through
+// wave_uniform / readfirstlane, so they sit in scalar registers and rows are
+// 32-bit offsets from base. The arithmetic is solve_chunk's.
+constexpr int LONG32_M = 32, LONG32_T = 256;
+
+template <typename REAL>
+constexpr size_t long32_lds_bytes() {
+  return sizeof(REAL) * (size_t)LONG32_T * (LONG32_M + 1) + ring_bytes<REAL>(LONG32_T, 3);
+}
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+// v of lane 0, which the compiler then holds in scalar registers (a value
+// every lane of the wave has).
+__device__ __forceinline__ int64_t wave_uniform(int64_t v) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)v >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// `fetched()` runs once b is in chunk order (after a barrier that follows
+// the loads); a persistent caller decodes its next item there.
+template <typename REAL, bool INC, typename OnFetched>
+__device__ __forceinline__ void long32_solve(int64_t base, int N, unsigned char *smem_raw,
+                                             const REAL *__restrict__ a,
+                                             const REAL *__restrict__ b,
+                                             const REAL *__restrict__ c,
+                                             REAL *__restrict__ d, REAL *__restrict__ u,
+                                             OnFetched fetched) {
+  constexpr int M = LONG32_M, T = LONG32_T;
+  const int t = threadIdx.x;
+  Lane L;
+  L.P = T; L.W = 1; L.T = T; L.t = t; L.p = t; L.w = 0; L.pstep = 1; L.stride = 1;
+  L.active = true; L.base = base; L.N = N; L.i0 = 0; L.i1 = 0;
+  REAL *tile = reinterpret_cast<REAL *>(smem_raw);
+  Ring ring{smem_raw + sizeof(REAL) * (size_t)T * (M + 1), (int)(3 * sizeof(REAL) * T), 0};
+  const REAL *sb = b + base, *sc = c + base, *sd = d + base;
+  // b, c, d in fetch order (lane t: rows t + T*r), from rows clamped into
+  // the system so that every load is unconditional; then identity rows past
+  // N and c of row N-1 set to their values
+  REAL rb[M], rc[M], rd[M];
+#pragma unroll
+  for (int r = 0; r < M; ++r) {
+    const int row = t + T * r;
+    const int rr = row < N ? row : N - 1;
+    rb[r] = sb[rr];
+    rc[r] = sc[rr];
+    rd[r] = sd[rr];
+  }
+#pragma unroll
+  for (int r = 0; r < M; ++r) {
+    const int row = t + T * r;
+    rb[r] = row < N ? rb[r] : REAL(1);
+    rc[r] = row < N - 1 ? rc[r] : REAL(0);
+    rd[r] = row < N ? rd[r] : REAL(0);
+  }
+  // transposes through the tile, hyb::to_chunk / to_fetch at P = T = 256,
+  // M = 32: fetch-order row t + 256r sits at slot skew<32>(t + 256r) =
+  // (t + t/32) + 264r, so each lane addresses all its slots from one base
+  // with constant offsets (no per-row address registers)
+  const int fslot = t + (t >> 5), cslot = t * (M + 1);
+  auto to_chunk = [&](REAL (&v)[M]) {
+#pragma unroll
+    for (int r = 0; r < M; ++r) tile[fslot + (T + T / M) * r] = v[r];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < M; ++r) v[r] = tile[cslot + r];
+    __syncthreads();
+  };
+  to_chunk(rb);
+  fetched();
+  to_chunk(rc);
+  to_chunk(rd);  // ends with a barrier: the tile is free
+  {
+    // wave v moves the 256-byte windows k = v, v + T/64, ...; lane l the
+    // dword l of window k (row k*E + l*E/64, E rows a window). Row 0's a is
+    // not used and rows >= N are masked off. (The thread index passes
+    // through an empty asm: the loop's per-lane values are formed here for
+    // each item rather than hoisted out of the caller's item loop and held
+    // across it.)
+    constexpr int E = 256 / (int)sizeof(REAL);  // rows of a 256-byte window
+    int tt = t;
+    asm volatile("" : "+v"(tt));
+    const int lane = tt & 63;
+    const int rl = lane * E / 64;
+    const char *src = reinterpret_cast<const char *>(a + base) + 4 * lane;
+    for (int k = tt >> 6; k * E < N; k += T / 64) {
+      const int row = k * E + rl;
+      const bool ok = row >= 1 && row < N;
+      const char *s = src + k * 256;
+      if constexpr (E == M) {  // FP64: the window is chunk k
+        if (ok) __builtin_amdgcn_global_load_lds(s, (lds_void *)(tile + k * (M + 1)), 4, 0, 0);
+      } else {  // FP32: chunks 2k and 2k+1
+        if (ok && lane < 32)
+          __builtin_amdgcn_global_load_lds(s, (lds_void *)(tile + 2 * k * (M + 1)), 4, 0, 0);
+        if (ok && lane >= 32)
+          __builtin_amdgcn_global_load_lds(s, (lds_void *)(tile + 2 * k * (M + 1) + 1), 4, 0, 0);
+      }
+    }
+  }
+  const int row0 = t * M;
+  REAL *al_s = tile + t * (M + 1);  // this lane's rows, chunk order
+  // a of the identity rows past N is 0: the lane clears its own slots, which
+  // no load writes
+  if (row0 + M > N) {
+#pragma unroll
+    for (int r = 0; r < M; ++r)
+      if (row0 + r >= N) al_s[r] = REAL(0);
+  }
+  // the tile is written by other waves' loads: wait for this wave's, then
+  // for every wave
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  REAL y, ym;
+  sweep_lds_a<REAL, M, 1>(L, ring, al_s, row0 == 0, rb, rc, rd, y, ym);
+  // (iv) substitution into rd; each lane reads only its own slots, which
+  // to_fetch then overwrites first
+#pragma unroll
+  for (int r = 0; r < M - 1; ++r) rd[r] = rd[r] - al_s[r] * ym - rc[r] * y;
+  rd[M - 1] = y;
+#pragma unroll
+  for (int r = 0; r < M; ++r) tile[cslot + r] = rd[r];
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < M; ++r) rd[r] = tile[fslot + (T + T / M) * r];
+  __syncthreads();
+  REAL *dst = (INC ? u : d) + base;
+#pragma unroll
+  for (int r = 0; r < M; ++r) {
+    const int row = t + T * r;
+    if (row < N) {
+      if (INC) dst[row] += rd[r];
+      else dst[row] = rd[r];
+    }
+  }
+}
+
 }  // namespace hyb
 }  // namespace trid

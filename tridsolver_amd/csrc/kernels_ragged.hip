@@ -281,163 +281,21 @@ __global__ __launch_bounds__(T) void rg_solve(
   }
 }
 
-typedef __attribute__((address_space(3))) void lds_void;
-
-// Systems needing more than 256 lanes at 16 rows per lane (4097-8192 rows)
-// at 32 rows per lane in 256 threads. b, c and d go to chunk order through
-// the LDS tile as in rg_item; `a` then goes from HBM into the same tile by
-// global->LDS loads, in chunk order, and stays there through the sweeps
-// (hyb::sweep_lds_a), so the lane holds 32 rows of b, c, d only (FP64: 256
-// VGPRs, two 256-thread workgroups per CU). The tile is skewed with one pad
-// slot per 32 rows (hyb::skew<32>, as the transposes use), so lane p reads
-// its rows at slots 33p + r. A global->LDS wave instruction writes 256
-// contiguous LDS bytes (lane l: bytes 4l .. 4l+3 from its own source):
-// FP64, rows 32k .. 32k+31 to slots 33k ..; FP32, rows 64j .. 64j+63 by two
-// instructions with complementary lane masks, lanes 0-31 (chunk 2j) to slots
-// 66j .. and lanes 32-63 (chunk 2j+1) from base slot 66j + 1, which puts row
-// 64j + l at slot 66j + 1 + l = 33(2j+1) + (l - 32). The items are single
-// systems (plan class 9 at Tmax 512).
-constexpr int RGL_M = 32, RGL_T = 256, RGL_PLAN_T = 512;
+// Systems needing more than 256 lanes at 16 rows per lane (4097-8192 rows):
+// hyb::long32_solve (32 rows per lane in 256 threads, `a` moved HBM->LDS in
+// chunk order). The items are single systems (plan class 9 at Tmax 512).
+constexpr int RGL_T = hyb::LONG32_T, RGL_PLAN_T = 512;
 
 // LDS of rg_solve_split: the long items' tile and ring (the rest need less).
 template <typename REAL>
 constexpr size_t rg_split_lds_bytes() {
-  return sizeof(REAL) * (size_t)RGL_T * (RGL_M + 1) + hyb::ring_bytes<REAL>(RGL_T, 3);
-}
-
-// v of lane 0, which the compiler then holds in scalar registers (a value
-// every lane of the wave has).
-__device__ __forceinline__ int64_t wave_uniform(int64_t v) {
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)v >> 32));
-  return (int64_t)(((uint64_t)hi << 32) | lo);
-}
-
-// `fetched()` as for rg_item.
-template <typename REAL, bool INC, typename OnFetched>
-__device__ __forceinline__ void rg_long_item(const RgMeta &cur, unsigned char *smem_raw,
-                                             const REAL *__restrict__ a,
-                                             const REAL *__restrict__ b,
-                                             const REAL *__restrict__ c,
-                                             REAL *__restrict__ d, REAL *__restrict__ u,
-                                             OnFetched fetched) {
-  constexpr int M = RGL_M, T = RGL_T;
-  const int t = threadIdx.x;
-  // the item's system (listed items are active): the same for every lane,
-  // so its base and length sit in scalar registers and rows are 32-bit
-  // offsets from it
-  const int64_t base = wave_uniform(cur.base);
-  const int N = __builtin_amdgcn_readfirstlane(cur.N);
-  hyb::Lane L;
-  L.P = T; L.W = 1; L.T = T; L.t = t; L.p = t; L.w = 0; L.pstep = 1; L.stride = 1;
-  L.active = true; L.base = base; L.N = N; L.i0 = 0; L.i1 = 0;
-  REAL *tile = reinterpret_cast<REAL *>(smem_raw);
-  hyb::Ring ring{smem_raw + sizeof(REAL) * (size_t)T * (M + 1), (int)(3 * sizeof(REAL) * T), 0};
-  const REAL *sb = b + base, *sc = c + base, *sd = d + base;
-  // b, c, d in fetch order (lane t: rows t + T*r), from rows clamped into
-  // the system so that every load is unconditional; then identity rows past
-  // N and c of row N-1 set to their values
-  REAL rb[M], rc[M], rd[M];
-#pragma unroll
-  for (int r = 0; r < M; ++r) {
-    const int row = t + T * r;
-    const int rr = row < N ? row : N - 1;
-    rb[r] = sb[rr];
-    rc[r] = sc[rr];
-    rd[r] = sd[rr];
-  }
-#pragma unroll
-  for (int r = 0; r < M; ++r) {
-    const int row = t + T * r;
-    rb[r] = row < N ? rb[r] : REAL(1);
-    rc[r] = row < N - 1 ? rc[r] : REAL(0);
-    rd[r] = row < N ? rd[r] : REAL(0);
-  }
-  // transposes through the tile, hyb::to_chunk / to_fetch at P = T = 256,
-  // M = 32: fetch-order row t + 256r sits at slot skew<32>(t + 256r) =
-  // (t + t/32) + 264r, so each lane addresses all its slots from one base
-  // with constant offsets (no per-row address registers)
-  const int fslot = t + (t >> 5), cslot = t * (M + 1);
-  auto to_chunk = [&](REAL (&v)[M]) {
-#pragma unroll
-    for (int r = 0; r < M; ++r) tile[fslot + (T + T / M) * r] = v[r];
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < M; ++r) v[r] = tile[cslot + r];
-    __syncthreads();
-  };
-  to_chunk(rb);
-  fetched();
-  to_chunk(rc);
-  to_chunk(rd);  // ends with a barrier: the tile is free
-  {
-    // wave v moves the 256-byte windows k = v, v + T/64, ...; lane l the
-    // dword l of window k (row k*E + l*E/64, E rows a window). Row 0's a is
-    // not used and rows >= N are masked off. (The thread index passes
-    // through an empty asm: the loop's per-lane values are formed here for
-    // each item rather than hoisted out of the caller's item loop and held
-    // across it.)
-    constexpr int E = 256 / (int)sizeof(REAL);  // rows of a 256-byte window
-    int tt = t;
-    asm volatile("" : "+v"(tt));
-    const int lane = tt & 63;
-    const int rl = lane * E / 64;
-    const char *src = reinterpret_cast<const char *>(a + base) + 4 * lane;
-    for (int k = tt >> 6; k * E < N; k += T / 64) {
-      const int row = k * E + rl;
-      const bool ok = row >= 1 && row < N;
-      const char *s = src + k * 256;
-      if constexpr (E == M) {  // FP64: the window is chunk k
-        if (ok) __builtin_amdgcn_global_load_lds(s, (lds_void *)(tile + k * (M + 1)), 4, 0, 0);
-      } else {  // FP32: chunks 2k and 2k+1
-        if (ok && lane < 32)
-          __builtin_amdgcn_global_load_lds(s, (lds_void *)(tile + 2 * k * (M + 1)), 4, 0, 0);
-        if (ok && lane >= 32)
-          __builtin_amdgcn_global_load_lds(s, (lds_void *)(tile + 2 * k * (M + 1) + 1), 4, 0, 0);
-      }
-    }
-  }
-  const int row0 = t * M;
-  REAL *al_s = tile + t * (M + 1);  // this lane's rows, chunk order
-  // a of the identity rows past N is 0: the lane clears its own slots, which
-  // no load writes
-  if (row0 + M > N) {
-#pragma unroll
-    for (int r = 0; r < M; ++r)
-      if (row0 + r >= N) al_s[r] = REAL(0);
-  }
-  // the tile is written by other waves' loads: wait for this wave's, then
-  // for every wave
-  __builtin_amdgcn_s_waitcnt(0);
-  __syncthreads();
-  REAL y, ym;
-  hyb::sweep_lds_a<REAL, M, 1>(L, ring, al_s, row0 == 0, rb, rc, rd, y, ym);
-  // (iv) substitution into rd; each lane reads only its own slots, which
-  // to_fetch then overwrites first
-#pragma unroll
-  for (int r = 0; r < M - 1; ++r) rd[r] = rd[r] - al_s[r] * ym - rc[r] * y;
-  rd[M - 1] = y;
-#pragma unroll
-  for (int r = 0; r < M; ++r) tile[cslot + r] = rd[r];
-  __syncthreads();
-#pragma unroll
-  for (int r = 0; r < M; ++r) rd[r] = tile[fslot + (T + T / M) * r];
-  __syncthreads();
-  REAL *dst = (INC ? u : d) + base;
-#pragma unroll
-  for (int r = 0; r < M; ++r) {
-    const int row = t + T * r;
-    if (row < N) {
-      if (INC) dst[row] += rd[r];
-      else dst[row] = rd[r];
-    }
-  }
+  return hyb::long32_lds_bytes<REAL>();
 }
 
 // The three-launch split's solve (plan at 16 rows per lane with Tmax 512 and
 // Tshort 256): one persistent launch of 256-thread workgroups over every
 // item (FP64: two per CU). The long items [0, long_items) come first in the
-// queue and take rg_long_item at 32 rows per lane; the rest take rg_item at
+// queue and take hyb::long32_solve at 32 rows per lane; the rest take rg_item at
 // 16 rows per lane. One launch, so a workgroup that runs out of long items
 // goes straight on to the short ones.
 template <typename REAL, bool INC>
@@ -469,7 +327,10 @@ __global__ __launch_bounds__(RGL_T, 2) void rg_solve_split(
       nxt = s_next;
       if (nxt < total) next = decode(nxt);
     };
-    if (item < nlong) rg_long_item<REAL, INC>(cur, smem_raw, a, b, c, d, u, fetched);
+    if (item < nlong)
+      hyb::long32_solve<REAL, INC>(hyb::wave_uniform(cur.base),
+                                   __builtin_amdgcn_readfirstlane(cur.N), smem_raw, a, b, c, d, u,
+                                   fetched);
     else rg_item<REAL, 16, INC, RGL_T>(cur, smem_raw, a, b, c, d, u, fetched);
     item = nxt;
     cur = next;

@@ -367,8 +367,10 @@ EXTERN_C size_t tridHipWorkspaceBytes(int fp64, int inc, int ndim, int solvedim,
 }
 
 EXTERN_C tridStatus_t tridHipLastConfig(int *algorithm, int *M, int *P, int *W, int *G) {
-  if (algorithm)
-    *algorithm = tl_last_algo == 3 && tl_last_v > 0 ? 5 + tl_last_v : tl_last_algo;
+  if (algorithm) {  // hybrid variants: 1-3 report 6-8, 4 reports 13
+    *algorithm = tl_last_algo == 3 && tl_last_v > 0 ? (tl_last_v == 4 ? 13 : 5 + tl_last_v)
+                                                    : tl_last_algo;
+  }
   if (M) *M = tl_last_m;
   if (P) *P = tl_last_p;
   if (W) *W = tl_last_w;
@@ -381,7 +383,7 @@ EXTERN_C tridStatus_t tridHipForceHybridConfig(int M, int P, int W, int G, int v
     tl_forced = HybridCfg();
     return TRID_STATUS_SUCCESS;
   }
-  if (M < 0 || P < 1 || W < 1 || G < 1 || variant < 0 || variant > 3)
+  if (M < 0 || P < 1 || W < 1 || G < 1 || variant < 0 || variant > 4)
     return TRID_STATUS_INVALID_VALUE;
   tl_forced.M = M;
   tl_forced.P = P;
