@@ -598,10 +598,11 @@ def bench_ragged(torch, api, tune):
 
 def bench_long(torch, api, tune):
     """Systems longer than one workgroup holds (16384 FP64 rows; at most 8192
-    fit the hybrid), which take the per-lane Thomas kernel: 8192 systems
-    along x (contiguous: each lane walks its own system) and along y
-    (strided: neighbouring lanes read neighbouring addresses). 40 B per point
-    algorithmic; the kernel also writes and reads back c' (72 B moved)."""
+    fit the hybrid): 8192 systems along x (contiguous) and along y (strided),
+    solved by the long-system path (hybrid segments reduced to one equation
+    per chunk, the reduced systems solved by a second batched solve, the
+    segments read again for the substitution). 40 B per point algorithmic,
+    about 85 B moved."""
     out = {}
     for name, sd, dims in (("x", 0, [16384, 8192, 1]), ("y", 1, [8192, 16384, 1])):
         n = dims[0] * dims[1]
