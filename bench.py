@@ -802,7 +802,10 @@ def main():
         with open(tpath) as f:
             tj = json.load(f)
         key = "fp64_%d_%s" % (n, dom)
-        if list(tj.get("configs", {}).get(key, [])) == list(cfgs[dom]):
+        shapes = tj.get("configs", {}).get(key, [])
+        if shapes and not isinstance(shapes[0], list):
+            shapes = [shapes]  # one profiled shape
+        if any(list(sh) == list(cfgs[dom]) for sh in shapes):
             traffic = tj.get(key)
         else:
             log("traffic: the profiled shape of %s differs from this run's %s; not reported"
