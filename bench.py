@@ -52,7 +52,7 @@ def parse():
     ap.add_argument("--cpu-slab", type=int, default=64,
                     help="z-extent of the CPU baseline sample (n x n x slab)")
     ap.add_argument("--verbose", action="store_true")
-    ap.add_argument("--access-dirs", default="yz", choices=["y", "z", "yz"],
+    ap.add_argument("--access-dirs", default="yz", choices=["x", "y", "z", "yz", "xyz"],
                     help="directions of --only access")
     ap.add_argument("--only", choices=["fp32", "mixed", "mixed-f32", "adjoint", "ragged", "adi",
                                        "partitioned", "cpu-extras", "oracle-checks", "access",
@@ -113,34 +113,36 @@ def measure_copy_bw(torch, lib, nbytes=4 << 30, reps=10):
 
 
 def bench_access(torch, api, tune, dirs="yz", n=512):
-    """Bandwidth ceiling of the strided solve's access pattern
-    (tridHipAccessProbe: the solve's loads and stores, an add per value in
-    place of its arithmetic) along y and z of an n^3 FP64 grid, for 128- and
-    256-byte row segments, 16 and 32 rows per lane and XCD groupings 1, 8,
-    16; 40 B per point as the solve. Beside it the solve of the same
-    direction (tuned for the geometry first with tune, else the default
-    shape). dirs: the directions to measure ("y", "z" or "yz")."""
+    """Bandwidth ceiling of the solve's access pattern (tridHipAccessProbe:
+    the solve's loads and stores, an add per value in place of its
+    arithmetic) along x, y and z of an n^3 FP64 grid: x at 4, 8 and 16 rows
+    per lane; y and z at 128- and 256-byte row segments and 16 and 32 rows
+    per lane; XCD groupings 1, 8, 16. 40 B per point as the solve. Beside it
+    the solve of the same direction (tuned for the geometry first with tune,
+    else the default shape). dirs: the directions to measure (any of "x",
+    "y", "z")."""
     lib = api._lib.lib()
     n3 = n ** 3
     dims = [n, n, n]
     a, b, c, d = gen_gpu(torch, n3, torch.float64, seed=512)
     out = {}
-    for sd, name in ((1, "y"), (2, "z")):
+    for sd, name in ((0, "x"), (1, "y"), (2, "z")):
         if name not in dirs:
             continue
         rows = {}
-        for seg in (128, 256):
-            for M in (16, 32):
-                for G in (1, 8, 16):
-                    st = lib.tridHipAccessProbe(api._ptr(a), api._ptr(b), api._ptr(c),
-                                                api._ptr(d), sd, api.int3(dims), seg, M, G)
-                    if st != 0:
-                        continue
-                    ms = event_ms(torch, lambda: lib.tridHipAccessProbe(
-                        api._ptr(a), api._ptr(b), api._ptr(c), api._ptr(d), sd,
-                        api.int3(dims), seg, M, G), reps=9)
-                    rows["seg%d_M%d_G%d" % (seg, M, G)] = {
-                        "ms": round(ms, 4), "alg_GBs": round(40.0 * n3 / ms / 1e6, 1)}
+        shapes = [(0, M) for M in (4, 8, 16)] if sd == 0 else \
+            [(seg, M) for seg in (128, 256) for M in (16, 32)]
+        for seg, M in shapes:
+            for G in (1, 8, 16):
+                st = lib.tridHipAccessProbe(api._ptr(a), api._ptr(b), api._ptr(c), api._ptr(d),
+                                            sd, api.int3(dims), seg, M, G)
+                if st != 0:
+                    continue
+                ms = event_ms(torch, lambda: lib.tridHipAccessProbe(
+                    api._ptr(a), api._ptr(b), api._ptr(c), api._ptr(d), sd, api.int3(dims), seg,
+                    M, G), reps=9)
+                key = "M%d_G%d" % (M, G) if sd == 0 else "seg%d_M%d_G%d" % (seg, M, G)
+                rows[key] = {"ms": round(ms, 4), "alg_GBs": round(40.0 * n3 / ms / 1e6, 1)}
         d.copy_(a)
         if tune:
             api.autotune(torch.float64, sd, dims)

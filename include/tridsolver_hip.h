@@ -264,7 +264,10 @@ EXTERN_C tridStatus_t tridHipStreamCopyShape(const void *src, void *dst, size_t 
  * back; G as the hybrid's XCD grouping. Reads 32 B and writes 8 B per point,
  * like the solve, with none of its arithmetic: the bandwidth that access
  * pattern allows. Grids below 4 GiB per array; workgroups of at most 1024
- * threads (512 at M = 32). Asynchronous on this thread's stream. */
+ * threads (512 at M = 32). solvedim 0 (seg_bytes 0, M = 4, 8 or 16): the
+ * contiguous solve's pattern, P = ceil(dims[0] / M) lanes of M rows per
+ * system (lane p: rows p + P*r), 256 / P systems per workgroup, every array
+ * read once and d written. Asynchronous on this thread's stream. */
 EXTERN_C tridStatus_t tridHipAccessProbe(const double *a, const double *b, const double *c,
                                          double *d, int solvedim, const int *dims, int seg_bytes,
                                          int M, int G);

@@ -147,8 +147,11 @@ def test_measurement_entry_points_reject_bad_arguments():
     assert lib.tridHipStreamCopyShape(p, p, 1 << 20, 256, 3, 1) == 3
     assert lib.tridHipStreamCopyShape(p, p, 1 << 20, 256, 8, 1) == 3
     assert lib.tridHipStreamCopyShape(p, p, 1000, 256, 1, 1) == 3
-    # access probe: strided directions only, 128- or 256-byte rows, M 16 or 32
+    # access probe: x with seg 0 and M 4, 8, 16; y, z with 128- or 256-byte rows and M 16 or 32
     assert lib.tridHipAccessProbe(p, p, p, p, 0, dims, 256, 32, 1) == 3
+    assert lib.tridHipAccessProbe(p, p, p, p, 0, dims, 0, 32, 1) == 3  # x: M = 4, 8, 16
+    assert lib.tridHipAccessProbe(p, p, p, p, 0, dims, 0, 16, 0) == 3
+    assert lib.tridHipAccessProbe(p, p, p, p, 3, dims, 0, 16, 1) == 3
     assert lib.tridHipAccessProbe(p, p, p, p, 2, dims, 512, 32, 1) == 3
     assert lib.tridHipAccessProbe(p, p, p, p, 2, dims, 256, 8, 1) == 3
     assert lib.tridHipAccessProbe(p, p, p, p, 2, dims, 256, 32, 0) == 3

@@ -153,6 +153,35 @@ hipError_t launch_probe(const Geom &g, int P, int G, const double *a, const doub
   return hipGetLastError();
 }
 
+// Contiguous systems (the x solve's pattern): W systems of P lanes in a
+// 256-thread workgroup, lane p holds rows p + P*r (r < M) of its system,
+// coalesced; reads a, b, c, d and writes d = a + b + c + d.
+template <int M>
+__global__ __launch_bounds__(256) void access_probe_x_kernel(Geom g, int P, int W, int G,
+                                                             const double *__restrict__ a,
+                                                             const double *__restrict__ b,
+                                                             const double *__restrict__ c,
+                                                             double *__restrict__ d) {
+  const hyb::Lane L = hyb::lane_of<true>(g, P, W, G);
+  double v[M];
+#pragma unroll
+  for (int r = 0; r < M; ++r) {
+    const int64_t row = hyb::fetch_row<M, true>(L, r);
+    const int64_t e = L.base + row;
+    v[r] = (L.active && row < L.N) ? a[e] + b[e] + c[e] + d[e] : 0.0;
+  }
+  hyb::store<M, true>(L, d, v);
+}
+
+template <int M>
+hipError_t launch_probe_x(const Geom &g, int P, int W, int G, const double *a, const double *b,
+                          const double *c, double *d, hipStream_t st) {
+  const int64_t blocks = (g.n0 + W - 1) / W * g.n1;
+  hipLaunchKernelGGL(access_probe_x_kernel<M>, dim3((unsigned)blocks), dim3(W * P), 0, st, g, P,
+                     W, G, a, b, c, d);
+  return hipGetLastError();
+}
+
 }  // namespace
 }  // namespace trid
 
@@ -190,8 +219,10 @@ EXTERN_C tridStatus_t tridHipStreamCopy(const void *src, void *dst, size_t bytes
 EXTERN_C tridStatus_t tridHipAccessProbe(const double *a, const double *b, const double *c,
                                          double *d, int solvedim, const int *dims, int seg_bytes,
                                          int M, int G) {
-  if (!a || !b || !c || !d || !dims || (solvedim != 1 && solvedim != 2) ||
-      (seg_bytes != 128 && seg_bytes != 256) || (M != 16 && M != 32) || G < 1)
+  if (!a || !b || !c || !d || !dims || solvedim < 0 || solvedim > 2 || G < 1)
+    return TRID_STATUS_INVALID_VALUE;
+  if (solvedim == 0 ? (seg_bytes != 0 || (M != 4 && M != 8 && M != 16))
+                    : ((seg_bytes != 128 && seg_bytes != 256) || (M != 16 && M != 32)))
     return TRID_STATUS_INVALID_VALUE;
   trid::Geom g;
   bool empty = false;
@@ -200,6 +231,17 @@ EXTERN_C tridStatus_t tridHipAccessProbe(const double *a, const double *b, const
   int dev = 0;
   if ((s = trid::check_device(&dev)) != TRID_STATUS_SUCCESS) return s;
   if (empty) return TRID_STATUS_SUCCESS;
+  if (solvedim == 0) {  // contiguous: P lanes of M rows per system, 256-thread workgroups
+    const int64_t P = (g.N + M - 1) / M;
+    if (P > 256 || g.n0 * g.n1 > (int64_t)0x7fffffff) return TRID_STATUS_INVALID_VALUE;
+    const int W = (int)std::max<int64_t>(1, 256 / P);
+    hipStream_t st = trid::current_stream();
+    hipError_t e;
+    if (M == 4) e = trid::launch_probe_x<4>(g, (int)P, W, G, a, b, c, d, st);
+    else if (M == 8) e = trid::launch_probe_x<8>(g, (int)P, W, G, a, b, c, d, st);
+    else e = trid::launch_probe_x<16>(g, (int)P, W, G, a, b, c, d, st);
+    return e == hipSuccess ? TRID_STATUS_SUCCESS : TRID_STATUS_EXECUTION_FAILED;
+  }
   const int W = seg_bytes / 8;
   const int per_wave = std::max(1, 64 / W);
   int P = (int)((g.N + M - 1) / M);
