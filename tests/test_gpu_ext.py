@@ -578,12 +578,14 @@ def test_ragged_split_launch(per_cu, dtype):
                              sync=True)
 
 
+@pytest.mark.parametrize("launches", [0, 1, 2])
 @pytest.mark.parametrize("max_len", [2048, 8192])
-def test_ragged_mixed_f32_vs_oracle(max_len):
+def test_ragged_mixed_f32_vs_oracle(max_len, launches):
     """FP64 solution of a ragged batch stored in FP32 (empty and one-row
     systems included): 1 and 2 refinements within 1e-12 of the FP64 oracle's
     solution of the promoted values, 0 refinements FP32-accurate; d is not
-    written and x keeps its value past the last system."""
+    written and x keeps its value past the last system. Default launches
+    (two where systems may exceed 4096 rows), or forced one or two."""
     rng = np.random.default_rng(max_len + 9)
     lens = np.round(np.exp(rng.uniform(np.log(1), np.log(max_len), 1500))).astype(np.int64)
     lens[:5] = [0, 1, 2, 0, max_len]
@@ -594,16 +596,24 @@ def test_ragged_mixed_f32_vs_oracle(max_len):
     oracle.ragged(*(v.astype(np.float64) for v in (a, b, c)), want, off)
     t = [dev(v) for v in (a, b, c, d)]
     toff = dev(off)
-    for refine, tol in ((0, 1e-5), (1, 1e-12), (2, 1e-12)):
-        x0 = torch.full((len(d),), 3.0, dtype=torch.float64, device="cuda")
-        x = api.ragged_batch_mixed_f32(*t, toff, x0, max_len=max_len, refinements=refine,
-                                       sync=True).cpu().numpy()
-        assert api.last_config()[0] == 11
-        n = int(off[-1])
-        err = H.assert_norm_close(x[:n], want[:n], np.float64, tol=tol)
-        if refine == 0:
-            assert err > 1e-12
-        assert np.all(x[n:] == 3.0)
+    want_launches = launches or (2 if max_len > 4096 else 1)
+    ctx = api.forced_ragged(16, launches=launches) if launches else None
+    if ctx:
+        ctx.__enter__()
+    try:
+        for refine, tol in ((0, 1e-5), (1, 1e-12), (2, 1e-12)):
+            x0 = torch.full((len(d),), 3.0, dtype=torch.float64, device="cuda")
+            x = api.ragged_batch_mixed_f32(*t, toff, x0, max_len=max_len, refinements=refine,
+                                           sync=True).cpu().numpy()
+            assert api.last_config()[:4:3] == (11, want_launches)
+            n = int(off[-1])
+            err = H.assert_norm_close(x[:n], want[:n], np.float64, tol=tol)
+            if refine == 0:
+                assert err > 1e-12
+            assert np.all(x[n:] == 3.0)
+    finally:
+        if ctx:
+            ctx.__exit__(None, None, None)
     assert np.array_equal(t[3].cpu().numpy(), d)
 
 
@@ -744,8 +754,8 @@ def test_config_ragged_40k(dtype):
 @pytest.mark.parametrize("refine", [1, 2])
 def test_config_ragged_40k_mixed(refine):
     """The 40,000-system ragged config (lengths round(exp(U(ln 8, ln 8192))))
-    by FP32 sweeps + FP64 refinement, one persistent launch: within 1e-12 of
-    the FP64 oracle."""
+    by FP32 sweeps + FP64 refinement, default launches (two: the systems over
+    4096 rows, then the rest): within 1e-12 of the FP64 oracle."""
     rng = np.random.default_rng(40000)
     lens = np.round(np.exp(rng.uniform(np.log(8), np.log(8192), 40000))).astype(np.int64)
     off, a, b, c, d = ragged_case(rng, lens, np.float64)
@@ -758,12 +768,13 @@ def test_config_ragged_40k_mixed(refine):
           % (refine, err))
 
 
-@pytest.mark.parametrize("M,launches", [(0, 1), (4, 1), (8, 1), (16, 1), (16, 2)])
+@pytest.mark.parametrize("M,launches", [(0, 2), (4, 1), (8, 1), (16, 1), (16, 2)])
 def test_ragged_mixed_vs_oracle(M, launches):
     """FP64 ragged batch by FP32 sweeps + FP64 refinement: within 1e-12 of the
     FP64 oracle after 2 refinements; 0 refinements is an FP32-accurate solve.
     M = 4 and 8 keep every operand in registers, M = 16 (the default) keeps
-    a and c in LDS, in one solve launch or split in two."""
+    a and c in LDS, in one solve launch or split in two (the default when
+    systems may exceed 4096 rows)."""
     rng = np.random.default_rng(40 + M)
     max_len = {4: 2048, 8: 4096}.get(M, 8192)
     lens = np.round(np.exp(rng.uniform(np.log(1), np.log(max_len), 2000))).astype(np.int64)

@@ -469,26 +469,29 @@ __global__ __launch_bounds__(RG_MIXED_THREADS) void rg_solve_mixed(
 // to fit the register file.
 constexpr int RG_M32_M = 16;
 
-template <bool ONE, bool AC_LDS>
-__global__ __launch_bounds__(RG_MIXED_THREADS) void rg_solve_mixed32(
+// T threads, LONG as in rg_solve (the range of a two-launch split).
+template <bool ONE, bool AC_LDS, int T, bool LONG>
+__global__ __launch_bounds__(T) void rg_solve_mixed32(
     const float *__restrict__ a, const float *__restrict__ b, const float *__restrict__ c,
     const float *__restrict__ d, double *__restrict__ x, const int64_t *__restrict__ off,
     const int *__restrict__ list, RaggedPlan *plan, int refine) {
-  constexpr int M = RG_M32_M, Tmax = RG_MIXED_THREADS;
+  constexpr int M = RG_M32_M, Tmax = T;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ int s_next;
   __shared__ RgPlanView s_view;
   rg_view_load(s_view, plan, threadIdx.x);
   const int t = threadIdx.x;
-  const int total = plan->total_items;
-  if (t == 0) s_next = atomicAdd(&plan->work, 1);
+  int *counter = LONG ? &plan->work : &plan->work_short;
+  const int lo = LONG ? 0 : plan->long_items;
+  const int total = LONG ? plan->long_items : plan->total_items;
+  if (t == 0) s_next = lo + atomicAdd(counter, 1);
   __syncthreads();
   int item = s_next;
   RgMeta cur = item < total ? rg_meta(s_view, list, off, item, t, Tmax) : RgMeta{};
   for (;;) {
     __syncthreads();  // every thread has read s_next
     if (item >= total) break;
-    if (t == 0) s_next = atomicAdd(&plan->work, 1);
+    if (t == 0) s_next = lo + atomicAdd(counter, 1);
     const hyb::Lane L = rg_lane(cur, t, Tmax);
     double *dtile = reinterpret_cast<double *>(smem_raw) + (size_t)L.w * L.P * (M + 1);
     float *ftile = reinterpret_cast<float *>(dtile);
@@ -527,27 +530,46 @@ __global__ __launch_bounds__(RG_MIXED_THREADS) void rg_solve_mixed32(
   }
 }
 
-template <bool ONE, bool AC_LDS>
-hipError_t launch_mixed32_one(const float *a, const float *b, const float *c, const float *d,
-                              double *x, const int64_t *off, int64_t S, int per_cu, int refine,
-                              void *ws, hipStream_t st, int dev) {
-  constexpr int Tmax = RG_MIXED_THREADS;
-  RaggedPlan *plan = static_cast<RaggedPlan *>(ws);
-  int *list = reinterpret_cast<int *>(static_cast<unsigned char *>(ws) + RAGGED_PLAN_HEADER_BYTES);
-  launch_plan(off, S, RG_M32_M, Tmax, Tmax, ws, st);  // one launch: no long range
-  const size_t lds =
-      sizeof(double) * (size_t)Tmax * (RG_M32_M + 1) + hyb::ring_bytes<double>(Tmax, 2);
-  auto kern = rg_solve_mixed32<ONE, AC_LDS>;
+template <bool ONE, bool AC_LDS, int T, bool LONG>
+hipError_t launch_mixed32_range(const float *a, const float *b, const float *c, const float *d,
+                                double *x, const int64_t *off, RaggedPlan *plan, int *list,
+                                unsigned blocks, int refine, hipStream_t st) {
+  const size_t lds = sizeof(double) * (size_t)T * (RG_M32_M + 1) + hyb::ring_bytes<double>(T, 2);
+  auto kern = rg_solve_mixed32<ONE, AC_LDS, T, LONG>;
   static LdsGrant lds_granted;
   {
     hipError_t e = ensure_lds(reinterpret_cast<const void *>(kern), lds, lds_granted);
     if (e != hipSuccess) return e;
   }
+  hipLaunchKernelGGL(kern, dim3(blocks), dim3(T), lds, st, a, b, c, d, x, off, list, plan,
+                     refine);
+  return hipGetLastError();
+}
+
+// One launch of 512-thread workgroups, or two: the systems needing more than
+// 256 lanes in 512-thread workgroups, then the rest in 256-thread ones, two
+// per CU.
+template <bool ONE, bool AC_LDS>
+hipError_t launch_mixed32_one(const float *a, const float *b, const float *c, const float *d,
+                              double *x, const int64_t *off, int64_t S, int per_cu,
+                              int launches, int refine, void *ws, hipStream_t st, int dev) {
+  constexpr int Tmax = RG_MIXED_THREADS, Tshort = Tmax / 2;
+  RaggedPlan *plan = static_cast<RaggedPlan *>(ws);
+  int *list = reinterpret_cast<int *>(static_cast<unsigned char *>(ws) + RAGGED_PLAN_HEADER_BYTES);
   int cus = 256;
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  hipLaunchKernelGGL(kern, dim3((unsigned)(cus * std::max(1, per_cu))), dim3(Tmax), lds, st,
-                     a, b, c, d, x, off, list, plan, refine);
-  return hipGetLastError();
+  const unsigned per = (unsigned)std::max(1, per_cu);
+  if (launches == 1) {  // no long range: the LONG = false launch takes every item
+    launch_plan(off, S, RG_M32_M, Tmax, Tmax, ws, st);
+    return launch_mixed32_range<ONE, AC_LDS, Tmax, false>(a, b, c, d, x, off, plan, list,
+                                                          cus * per, refine, st);
+  }
+  launch_plan(off, S, RG_M32_M, Tmax, Tshort, ws, st);
+  hipError_t e = launch_mixed32_range<ONE, AC_LDS, Tmax, true>(a, b, c, d, x, off, plan, list,
+                                                               cus * per, refine, st);
+  if (e != hipSuccess) return e;
+  return launch_mixed32_range<ONE, AC_LDS, Tshort, false>(a, b, c, d, x, off, plan, list,
+                                                          2 * cus * per, refine, st);
 }
 
 template <int M>
@@ -671,10 +693,12 @@ int ragged_mixed32_max_len() { return RG_M32_M * RG_MIXED_THREADS; }
 
 hipError_t launch_ragged_mixed32(const float *a, const float *b, const float *c, const float *d,
                                  double *x, const int64_t *off, int64_t S, int per_cu,
-                                 int refine, void *ws, hipStream_t st, int dev) {
-  return refine <= 1
-             ? launch_mixed32_one<true, false>(a, b, c, d, x, off, S, per_cu, refine, ws, st, dev)
-             : launch_mixed32_one<false, true>(a, b, c, d, x, off, S, per_cu, refine, ws, st, dev);
+                                 int launches, int refine, void *ws, hipStream_t st, int dev) {
+  if (launches != 1 && launches != 2) return hipErrorInvalidValue;
+  return refine <= 1 ? launch_mixed32_one<true, false>(a, b, c, d, x, off, S, per_cu, launches,
+                                                       refine, ws, st, dev)
+                     : launch_mixed32_one<false, true>(a, b, c, d, x, off, S, per_cu, launches,
+                                                       refine, ws, st, dev);
 }
 
 int ragged_mixed_max_len(int M) {

@@ -137,7 +137,7 @@ int ragged_mixed_max_len(int M);
 int ragged_mixed32_max_len();
 hipError_t launch_ragged_mixed32(const float *a, const float *b, const float *c, const float *d,
                                  double *x, const int64_t *off, int64_t S, int per_cu,
-                                 int refine, void *ws, hipStream_t st, int dev);
+                                 int launches, int refine, void *ws, hipStream_t st, int dev);
 hipError_t launch_ragged_mixed(int M, const double *a, const double *b, const double *c,
                                double *d, const int64_t *off, int64_t S, int per_cu,
                                int launches, int refine, void *ws, hipStream_t st, int dev);
