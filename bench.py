@@ -604,9 +604,12 @@ def bench_long(torch, api, tune):
     solved by the long-system path (hybrid segments reduced to one equation
     per chunk, the reduced systems solved by a second batched solve, the
     segments read again for the substitution). 40 B per point algorithmic,
-    about 85 B moved."""
+    about 85 B moved. Beside them, x8192: 8192 contiguous systems of 8192 rows
+    (the largest one workgroup holds, one system per workgroup at 32 rows per
+    lane), 40 B per point moved once."""
     out = {}
-    for name, sd, dims in (("x", 0, [16384, 8192, 1]), ("y", 1, [8192, 16384, 1])):
+    for name, sd, dims in (("x", 0, [16384, 8192, 1]), ("y", 1, [8192, 16384, 1]),
+                           ("x8192", 0, [8192, 8192, 1])):
         n = dims[0] * dims[1]
         a, b, c, d = gen_gpu(torch, n, torch.float64, seed=16384)
         if tune:
