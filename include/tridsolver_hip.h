@@ -37,8 +37,8 @@ EXTERN_C tridStatus_t tridHipReserveWorkspace(size_t bytes);
 /* Frees every workspace of the current device (synchronises the device). */
 EXTERN_C tridStatus_t tridHipReleaseWorkspaces(void);
 /* Workspace bytes one call may need: the per-lane path's, or, for systems
- * longer than one workgroup holds (above 8192 contiguous / 2048 strided FP64
- * rows), the larger of that and the long-system path's. */
+ * longer than one workgroup holds (above 16384 contiguous / 2048 strided
+ * FP64 rows), the larger of that and the long-system path's. */
 EXTERN_C size_t tridHipWorkspaceBytes(int fp64, int inc, int ndim, int solvedim,
                                       const int *dims);
 
@@ -51,7 +51,7 @@ EXTERN_C size_t tridHipWorkspaceBytes(int fp64, int inc, int ndim, int solvedim,
  * 9 ragged mixed precision, 10 FP64 solution of FP32-stored systems, 11 the
  * same for a ragged batch, 12 systems longer than one workgroup holds: the
  * hybrid with its reduced system solved by a second batched solve, 13
- * contiguous systems of 4097-8192 rows, one per workgroup, with a in LDS),
+ * contiguous systems of 4097-16384 rows, one per workgroup, with a in LDS),
  * and
  * for the hybrid its
  * rows per lane M, lanes per system P, systems per workgroup W and XCD
@@ -65,9 +65,9 @@ EXTERN_C tridStatus_t tridHipLastConfig(int *algorithm, int *M, int *P, int *W, 
  * value per row kept in LDS (arrays spanning < 4 GiB); 3: strided variant
  * with a staged in LDS by global->LDS loads (x-neighbouring strided systems,
  * W * sizeof = 128 or 256 bytes, M = 16/32, W * P a multiple of 64, arrays
- * spanning < 4 GiB); 4: contiguous systems of 4097-8192 rows, one per
- * 256-thread workgroup, with a moved into LDS in chunk order by global->LDS
- * loads (M = 32, P = 256, W = 1). A shape that does not
+ * spanning < 4 GiB); 4: contiguous systems, one per workgroup of P threads,
+ * with a moved into LDS in chunk order by global->LDS loads (M = 32, W = 1;
+ * P = 256 for 4097-8192 rows, P = 512 for 8193-16384). A shape that does not
  * fit a call (P*M < length, too many threads or LDS) makes that call return
  * TRID_STATUS_INVALID_VALUE. */
 EXTERN_C tridStatus_t tridHipForceHybridConfig(int M, int P, int W, int G, int variant);

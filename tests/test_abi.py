@@ -108,18 +108,23 @@ def test_workspace_bytes():
 
 
 def test_workspace_bytes_long_systems():
-    # contiguous FP64 systems of up to 8192 rows fit one workgroup: only the
+    # contiguous FP64 systems of up to 16384 rows fit one workgroup: only the
     # per-lane path's workspace (N * nsys reals)
     lib = _lib.lib()
     assert lib.tridHipWorkspaceBytes(1, 0, 2, 0, _lib.int3([8192, 3])) == 8 * 8192 * 3
-    # 9000 rows: two 8192-row segments of 512 chunks each, 10 reals per chunk
-    # (6 records, 4 reduced arrays); the larger of that and the lane path's
-    assert lib.tridHipWorkspaceBytes(1, 0, 2, 0, _lib.int3([9000, 3])) == \
-        max(8 * 9000 * 3, 8 * 10 * 1024 * 3)
-    # 140000 rows: 18 segments -> 9216-row reduced systems, themselves too
-    # long: a second level of 2 segments (1024 chunks)
+    assert lib.tridHipWorkspaceBytes(1, 0, 2, 0, _lib.int3([16384, 3])) == 8 * 16384 * 3
+    # 17000 rows: three 8192-row segments of 512 chunks each, 10 reals per
+    # chunk (6 records, 4 reduced arrays); the larger of that and the lane
+    # path's
+    assert lib.tridHipWorkspaceBytes(1, 0, 2, 0, _lib.int3([17000, 3])) == \
+        max(8 * 17000 * 3, 8 * 10 * 1536 * 3)
+    # 140000 rows: 18 segments -> 9216-row reduced systems, which fit
     assert lib.tridHipWorkspaceBytes(1, 0, 2, 0, _lib.int3([140000, 2])) == \
-        max(8 * 140000 * 2, 8 * 10 * (9216 * 2 + 1024 * 2))
+        max(8 * 140000 * 2, 8 * 10 * 9216 * 2)
+    # 270000 rows: 33 segments -> 16896-row reduced systems, themselves too
+    # long: a second level of 3 segments (1536 chunks)
+    assert lib.tridHipWorkspaceBytes(1, 0, 2, 0, _lib.int3([270000, 2])) == \
+        max(8 * 270000 * 2, 8 * 10 * (16896 * 2 + 1536 * 2))
     # strided FP64: up to 2048 rows in one workgroup; 5000 rows: 10 segments
     # of 512 rows, 32 chunks each
     assert lib.tridHipWorkspaceBytes(1, 0, 3, 1, _lib.int3([17, 2048, 2])) == 8 * 17 * 2048 * 2

@@ -1,4 +1,4 @@
-"""Systems longer than one workgroup holds (above 8192 contiguous / 2048
+"""Systems longer than one workgroup holds (above 16384 contiguous / 2048
 strided FP64 rows): the hybrid with its reduced system solved by a second
 batched solve (kernels_long.hip), against the oracle on the same inputs --
 normwise <= 1e-12 FP64 / 1e-5 FP32 and the reference tests' elementwise
@@ -30,11 +30,25 @@ def _gpu():
     torch.cuda.set_device(0)
 
 
-# (dims, solvedim): just past one workgroup, several segments, and lengths
-# whose reduced systems are themselves too long (a second level)
-CASES = [([8193, 3], 0), ([20000, 5, 2], 0), ([140000, 2], 0),
+# (dims, solvedim): around the longest system one workgroup holds, several
+# segments, and lengths whose reduced systems are themselves too long (a
+# second level: more than 16384 reduced rows, 33 segments of 8192 rows)
+CASES = [([8193, 3], 0), ([16385, 3], 0), ([20000, 5, 2], 0), ([140000, 2], 0),
+         ([270000, 2], 0),
          ([40, 2049, 3], 1), ([17, 5000, 2], 1), ([8, 40000], 1),
          ([20, 3, 3000], 2), ([33, 2, 9000], 2)]
+
+
+def _expected_algo(n, sd, fp64):
+    """tridHipLastConfig's algorithm for an untuned solve of n-row systems:
+    contiguous, one workgroup holds 16384 rows (FP64 above 4096 rows at 32
+    rows per lane, algorithm 13; FP32 at 16 rows per lane on up to 1024
+    lanes, 3); strided, 16 rows per lane on 128 (FP64) or 256 (FP32) lanes."""
+    if sd == 0:
+        if n > 16384:
+            return LONG
+        return 13 if fp64 and n > 4096 else 3
+    return LONG if n > (2048 if fp64 else 4096) else 3
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
@@ -42,18 +56,14 @@ CASES = [([8193, 3], 0), ([20000, 5, 2], 0), ([140000, 2], 0),
 @pytest.mark.parametrize("dims,sd", CASES)
 def test_long_vs_oracle(dims, sd, inc, dtype):
     cfg = run_case(dims, sd, dtype, inc, (0, 0, 0), seed=sum(dims) + sd)
-    # longest system one workgroup holds: 16 rows per lane on 512 (FP64) or
-    # 1024 (FP32) lanes contiguous, 128 lanes strided
-    fp64 = dtype == np.float64
-    one_wg = (8192 if fp64 else 16384) if sd == 0 else (2048 if fp64 else 4096)
-    assert cfg[0] == (LONG if dims[sd] > one_wg else 3), cfg
+    assert cfg[0] == _expected_algo(dims[sd], sd, dtype == np.float64), cfg
 
 
 @pytest.mark.parametrize("sd", [0, 1, 2])
 def test_long_padded(sd):
     """Halo-padded arrays, pointers offset by one halo layer: the padding is
     untouched and the interior matches the oracle."""
-    dims = [[9000, 3, 2], [5, 3000, 2], [5, 2, 3000]][sd]
+    dims = [[17000, 3, 2], [5, 3000, 2], [5, 2, 3000]][sd]
     rng = np.random.default_rng(90 + sd)
     n = int(np.prod(dims))
     a, b, c, d = H.diag_dominant(rng, n)
@@ -84,7 +94,7 @@ def test_long_opts_keep_the_lane_path():
 def test_long_graph_replay(sd):
     """Captured once after reserving tridHipWorkspaceBytes for the capture
     stream; replays with eager work in between equal the eager solve."""
-    dims = [[12000, 4], [16, 6000]][sd]
+    dims = [[20000, 4], [16, 6000]][sd]
     rng = np.random.default_rng(60 + sd)
     n = int(np.prod(dims))
     a, b, c, d = (torch.from_numpy(t).cuda() for t in H.diag_dominant(rng, n))

@@ -213,18 +213,42 @@ def test_forced_long32_contiguous(G, inc, dtype):
             run_case([4096, 3], 0, dtype, inc, (0, 0, 0), seed=1)
         with pytest.raises(Exception):
             run_case([3, 5000], 1, dtype, inc, (0, 0, 0), seed=2)
+        with pytest.raises(Exception):
+            run_case([8193, 2], 0, dtype, inc, (0, 0, 0), seed=3)
 
 
-def test_forced_long32_padded():
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("inc", [False, True])
+@pytest.mark.parametrize("G", [1, 8])
+def test_forced_long32_512_contiguous(G, inc, dtype):
+    """Contiguous systems of 8193-16384 rows, one per 512-thread workgroup at
+    32 rows per lane (variant 4 at P = 512, 156 KiB of LDS at FP64): lengths
+    at both ends of the range and between, Inc; refused for 8192 rows and
+    fewer and for strided systems."""
+    for dims in ([8193, 3], [10001, 2, 2], [16384, 3], [12345, 2]):
+        with api.forced_hybrid(32, 512, 1, G, 4):
+            cfg = run_case(dims, 0, dtype, inc, (0, 0, 0), seed=dims[0] + G)
+        assert cfg == (13, 32, 512, 1, G)
+    with api.forced_hybrid(32, 512, 1, G, 4):
+        with pytest.raises(Exception):
+            run_case([8192, 3], 0, dtype, inc, (0, 0, 0), seed=1)
+        with pytest.raises(Exception):
+            run_case([16385, 2], 0, dtype, inc, (0, 0, 0), seed=2)
+        with pytest.raises(Exception):
+            run_case([3, 9000], 1, dtype, inc, (0, 0, 0), seed=3)
+
+
+@pytest.mark.parametrize("P,n0", [(256, 4500), (512, 9001)])
+def test_forced_long32_padded(P, n0):
     """Variant 4 on the interior of a halo-padded allocation (row pitch and
     plane pitch not multiples of anything): the halo is untouched."""
     rng = np.random.default_rng(29)
-    dims = [4500, 3, 2]
+    dims = [n0, 3, 2]
     n = int(np.prod(dims))
     a, b, c, d = H.diag_dominant(rng, n)
     arrs = [H.pad_halo(t, dims) for t in (a, b, c, d)]
     off, pads = H.halo_offset(dims), [t + 2 for t in dims]
-    with api.forced_hybrid(32, 256, 1, 8, 4):
+    with api.forced_hybrid(32, P, 1, 8, 4):
         gd, _ = gpu_solve(*arrs, solvedim=0, dims=dims, pads=pads, offset=off)
         assert api.last_config()[0] == 13
     od = arrs[3].copy()

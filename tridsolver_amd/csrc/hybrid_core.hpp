@@ -540,28 +540,32 @@ __device__ __forceinline__ void solve_chunk(const Lane &L, Ring &ring, TR (&ra)[
   substitute<TR, M>(ra, rc, rd, y, ym, x);
 }
 
-// ---- one system of 4097-8192 rows in one 256-thread workgroup -----------
-// 32 rows per lane, 256 lanes. b, c and d go to chunk order through the LDS
-// tile; `a` then goes from HBM into the same tile by global->LDS loads, in
-// chunk order, and stays there through the sweeps (sweep_lds_a), so the lane
-// holds 32 rows of b, c, d only (FP64: about 256 VGPRs, two workgroups per
-// CU). The tile is skewed with one pad slot per 32 rows (skew<32>, as the
-// transposes use), so lane p reads its rows at slots 33p + r. A
-// global->LDS wave instruction writes 256 contiguous LDS bytes (lane l:
-// bytes 4l .. 4l+3 from its own source): FP64, rows 32k .. 32k+31 to slots
-// 33k ..; FP32, rows 64j .. 64j+63 by two instructions with complementary
-// lane masks, lanes 0-31 (chunk 2j) to slots 66j .. and lanes 32-63 (chunk
-// 2j+1) from base slot 66j + 1, which puts row 64j + l at slot 66j + 1 + l =
-// 33(2j+1) + (l - 32). base and N are the system's first element and length
-// (4096 < N <= 8192), the same for every lane: callers pass them through
-// wave_uniform / readfirstlane, so they sit in scalar registers and rows are
-// 32-bit offsets from base. The arithmetic is solve_chunk's.
-constexpr int LONG32_M = 32, LONG32_T = 256;
+// ---- one long system in one workgroup, 32 rows per lane -----------------
+// 32 rows per lane, T lanes: systems of 4097-8192 rows in 256 threads, 8193-
+// 16384 in 512. b, c and d go to chunk order through the LDS tile; `a` then
+// goes from HBM into the same tile by global->LDS loads, in chunk order, and
+// stays there through the sweeps (sweep_lds_a), so the lane holds 32 rows of
+// b, c, d only (FP64: about 256 VGPRs, two waves per SIMD: two 256-thread
+// workgroups per CU or one of 512). The tile is skewed with one pad slot per
+// 32 rows (skew<32>, as the transposes use), so lane p reads its rows at
+// slots 33p + r. A global->LDS wave instruction writes 256 contiguous LDS
+// bytes (lane l: bytes 4l .. 4l+3 from its own source): FP64, rows 32k ..
+// 32k+31 to slots 33k ..; FP32, rows 64j .. 64j+63 by two instructions with
+// complementary lane masks, lanes 0-31 (chunk 2j) to slots 66j .. and lanes
+// 32-63 (chunk 2j+1) from base slot 66j + 1, which puts row 64j + l at slot
+// 66j + 1 + l = 33(2j+1) + (l - 32). base and N are the system's first
+// element and length (N <= 32 T), the same for every lane: callers pass them
+// through wave_uniform / readfirstlane, so they sit in scalar registers and
+// rows are 32-bit offsets from base. The arithmetic is solve_chunk's.
+constexpr int LONG32_M = 32, LONG32_T = 256, LONG32_T_MAX = 512;
 
-template <typename REAL>
+// LDS of long32_solve at T lanes: the tile and the reduced solve's ring
+// (FP64 at 512 lanes: 156 KiB)
+template <typename REAL, int T = LONG32_T>
 constexpr size_t long32_lds_bytes() {
-  return sizeof(REAL) * (size_t)LONG32_T * (LONG32_M + 1) + ring_bytes<REAL>(LONG32_T, 3);
+  return sizeof(REAL) * (size_t)T * (LONG32_M + 1) + ring_bytes<REAL>(T, 3);
 }
+static_assert(long32_lds_bytes<double, LONG32_T_MAX>() <= 160 * 1024, "LDS of one CU");
 
 typedef __attribute__((address_space(3))) void lds_void;
 
@@ -575,14 +579,15 @@ __device__ __forceinline__ int64_t wave_uniform(int64_t v) {
 
 // `fetched()` runs once b is in chunk order (after a barrier that follows
 // the loads); a persistent caller decodes its next item there.
-template <typename REAL, bool INC, typename OnFetched>
+template <typename REAL, bool INC, int T, typename OnFetched>
 __device__ __forceinline__ void long32_solve(int64_t base, int N, unsigned char *smem_raw,
                                              const REAL *__restrict__ a,
                                              const REAL *__restrict__ b,
                                              const REAL *__restrict__ c,
                                              REAL *__restrict__ d, REAL *__restrict__ u,
                                              OnFetched fetched) {
-  constexpr int M = LONG32_M, T = LONG32_T;
+  constexpr int M = LONG32_M;
+  static_assert(T % 64 == 0 && T <= LONG32_T_MAX, "whole waves, 32 T rows at most");
   const int t = threadIdx.x;
   Lane L;
   L.P = T; L.W = 1; L.T = T; L.t = t; L.p = t; L.w = 0; L.pstep = 1; L.stride = 1;
@@ -609,10 +614,10 @@ __device__ __forceinline__ void long32_solve(int64_t base, int N, unsigned char 
     rc[r] = row < N - 1 ? rc[r] : REAL(0);
     rd[r] = row < N ? rd[r] : REAL(0);
   }
-  // transposes through the tile, hyb::to_chunk / to_fetch at P = T = 256,
-  // M = 32: fetch-order row t + 256r sits at slot skew<32>(t + 256r) =
-  // (t + t/32) + 264r, so each lane addresses all its slots from one base
-  // with constant offsets (no per-row address registers)
+  // transposes through the tile, hyb::to_chunk / to_fetch at P = T, M = 32:
+  // fetch-order row t + Tr sits at slot skew<32>(t + Tr) = (t + t/32) +
+  // (T + T/32)r, so each lane addresses all its slots from one base with
+  // constant offsets (no per-row address registers)
   const int fslot = t + (t >> 5), cslot = t * (M + 1);
   auto to_chunk = [&](REAL (&v)[M]) {
 #pragma unroll

@@ -328,7 +328,7 @@ __global__ __launch_bounds__(RGL_T, 2) void rg_solve_split(
       if (nxt < total) next = decode(nxt);
     };
     if (item < nlong)
-      hyb::long32_solve<REAL, INC>(hyb::wave_uniform(cur.base),
+      hyb::long32_solve<REAL, INC, RGL_T>(hyb::wave_uniform(cur.base),
                                    __builtin_amdgcn_readfirstlane(cur.N), smem_raw, a, b, c, d, u,
                                    fetched);
     else rg_item<REAL, 16, INC, RGL_T>(cur, smem_raw, a, b, c, d, u, fetched);

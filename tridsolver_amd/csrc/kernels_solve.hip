@@ -211,33 +211,43 @@ hipError_t launch_hybrid_m(const Geom &g, const HybridCfg &cfg, const REAL *a,
   return hipGetLastError();
 }
 
-// Contiguous systems of 4097-8192 rows, one per 256-thread workgroup
-// (variant 4): hyb::long32_solve, 32 rows per lane with `a` moved HBM->LDS in
-// chunk order, two workgroups per CU at FP64. The workgroups are dealt to
-// the XCDs in G groups as the other kernels' are.
-template <typename REAL, bool INC>
-__global__ __launch_bounds__(hyb::LONG32_T, 2) void hybrid_long32_kernel(
+// Contiguous systems of 4097-8192 rows, one per 256-thread workgroup, and of
+// 8193-16384 rows, one per 512-thread workgroup (variant 4, P = T):
+// hyb::long32_solve, 32 rows per lane with `a` moved HBM->LDS in chunk order;
+// at FP64 two waves per SIMD (two 256-thread workgroups per CU, or one of
+// 512). The workgroups are dealt to the XCDs in G groups as the other
+// kernels' are.
+template <typename REAL, bool INC, int T>
+__global__ __launch_bounds__(T, 512 / T) void hybrid_long32_kernel(
     Geom g, int G, const REAL *__restrict__ a, const REAL *__restrict__ b,
     const REAL *__restrict__ c, REAL *__restrict__ d, REAL *__restrict__ u) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const uint32_t n0 = (uint32_t)g.n0;
   const uint32_t sys = hyb::xcd_remap(blockIdx.x, gridDim.x, G);
   const int64_t base = (int64_t)(sys % n0) * g.st0 + (int64_t)(sys / n0) * g.st1;
-  hyb::long32_solve<REAL, INC>(hyb::wave_uniform(base), (int)g.N, smem_raw, a, b, c, d, u,
-                               [] {});
+  hyb::long32_solve<REAL, INC, T>(hyb::wave_uniform(base), (int)g.N, smem_raw, a, b, c, d, u,
+                                  [] {});
+}
+
+template <typename REAL, bool INC, int T>
+hipError_t launch_long32_t(const Geom &g, const HybridCfg &cfg, const REAL *a, const REAL *b,
+                           const REAL *c, REAL *d, REAL *u, hipStream_t st) {
+  auto kern = hybrid_long32_kernel<REAL, INC, T>;
+  static LdsGrant lds_granted;
+  constexpr size_t lds = hyb::long32_lds_bytes<REAL, T>();
+  hipError_t e = ensure_lds(reinterpret_cast<const void *>(kern), lds, lds_granted);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(kern, dim3((unsigned)(g.n0 * g.n1)), dim3(T), lds, st, g, cfg.G, a, b, c,
+                     d, u);
+  return hipGetLastError();
 }
 
 template <typename REAL, bool INC>
 hipError_t launch_long32(const Geom &g, const HybridCfg &cfg, const REAL *a, const REAL *b,
                          const REAL *c, REAL *d, REAL *u, hipStream_t st) {
-  auto kern = hybrid_long32_kernel<REAL, INC>;
-  static LdsGrant lds_granted;
-  constexpr size_t lds = hyb::long32_lds_bytes<REAL>();
-  hipError_t e = ensure_lds(reinterpret_cast<const void *>(kern), lds, lds_granted);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(kern, dim3((unsigned)(g.n0 * g.n1)), dim3(hyb::LONG32_T), lds, st, g,
-                     cfg.G, a, b, c, d, u);
-  return hipGetLastError();
+  if (cfg.P == hyb::LONG32_T_MAX)
+    return launch_long32_t<REAL, INC, hyb::LONG32_T_MAX>(g, cfg, a, b, c, d, u, st);
+  return launch_long32_t<REAL, INC, hyb::LONG32_T>(g, cfg, a, b, c, d, u, st);
 }
 
 }  // namespace
@@ -271,9 +281,13 @@ template <typename REAL>
 bool hybrid_cfg_valid(const Geom &g, const HybridCfg &cfg) {
   if (cfg.V == 1 || cfg.V == 2) return stream_cfg_valid<REAL>(g, cfg);
   if (cfg.V == 3) return wide_cfg_valid<REAL>(g, cfg);
-  if (cfg.V == 4)  // one contiguous system of 4097-8192 rows per 256-thread workgroup
-    return g.stride == 1 && cfg.M == hyb::LONG32_M && cfg.P == hyb::LONG32_T && cfg.W == 1 &&
-           cfg.G >= 1 && g.N > 4096 && g.N <= (int64_t)hyb::LONG32_M * hyb::LONG32_T;
+  if (cfg.V == 4) {  // one contiguous system per workgroup: P = 256 for 4097-8192
+                     // rows, 512 for 8193-16384
+    const bool p256 = cfg.P == hyb::LONG32_T, p512 = cfg.P == hyb::LONG32_T_MAX;
+    return g.stride == 1 && cfg.M == hyb::LONG32_M && cfg.W == 1 && cfg.G >= 1 &&
+           ((p256 && g.N > 4096) || (p512 && g.N > (int64_t)hyb::LONG32_M * hyb::LONG32_T)) &&
+           g.N <= (int64_t)hyb::LONG32_M * cfg.P;
+  }
   if (cfg.V != 0) return false;
   const int maxT = max_threads_for_m<REAL>(cfg.M);
   if (maxT == 0 || cfg.P < 1 || cfg.W < 1) return false;
@@ -350,10 +364,12 @@ bool hybrid_default_cfg(const Geom &g, HybridCfg *cfg) {
   // / 4.3 / 4.7 TB/s against 3.2 / 3.7 / 4.0 with 512 lanes of 16 rows), FP32
   // up to 6144 rows (4.0 / 4.4 against 3.2 / 3.8; at 8192 rows 4.4 against
   // 4.7).
+  // FP64 systems of 8193-16384 rows, longer than the other shapes hold, one
+  // per 512-thread workgroup.
   if (g.stride == 1 && g.N > 4096 && (sizeof(REAL) == 8 || g.N <= 6144)) {
     HybridCfg c;
     c.M = hyb::LONG32_M;
-    c.P = hyb::LONG32_T;
+    c.P = g.N > (int64_t)hyb::LONG32_M * hyb::LONG32_T ? hyb::LONG32_T_MAX : hyb::LONG32_T;
     c.W = 1;
     c.G = sizeof(REAL) == 8 ? 8 : 1;
     c.V = 4;

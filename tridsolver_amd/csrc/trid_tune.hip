@@ -178,12 +178,14 @@ std::vector<HybridCfg> candidates(const Geom &g) {
           }
         }
       }
-  if (g.stride == 1 && g.N > 4096) {  // one system of up to 8192 rows per workgroup, a in LDS
-    for (int G : {1, 8}) {
-      HybridCfg c;
-      c.M = 32; c.P = 256; c.W = 1; c.G = G; c.V = 4;
-      if (hybrid_cfg_valid<REAL>(g, c)) out.push_back(c);
-    }
+  if (g.stride == 1 && g.N > 4096) {  // one system of up to 8192 (256 lanes) or 16384
+                                       // rows (512 lanes) per workgroup, a in LDS
+    for (int P : {256, 512})
+      for (int G : {1, 8}) {
+        HybridCfg c;
+        c.M = 32; c.P = P; c.W = 1; c.G = G; c.V = 4;
+        if (hybrid_cfg_valid<REAL>(g, c)) out.push_back(c);
+      }
   }
   if (g.stride != 1) {  // wide variant: 128- or 256-byte rows, a in LDS; finer XCD groupings
     for (int seg : {128, 256})
