@@ -599,14 +599,14 @@ def bench_ragged(torch, api, tune):
 
 
 def bench_long(torch, api, tune):
-    """Systems longer than one workgroup holds (16384 FP64 rows; at most 8192
-    fit the hybrid): 8192 systems along x (contiguous) and along y (strided),
-    solved by the long-system path (hybrid segments reduced to one equation
-    per chunk, the reduced systems solved by a second batched solve, the
-    segments read again for the substitution). 40 B per point algorithmic,
-    about 85 B moved. Beside them, x8192: 8192 contiguous systems of 8192 rows
-    (the largest one workgroup holds, one system per workgroup at 32 rows per
-    lane), 40 B per point moved once."""
+    """Long FP64 systems, 40 B per point algorithmic. x: 8192 contiguous
+    systems of 16384 rows, the longest one workgroup holds (512 lanes of 32
+    rows, 40 B per point moved once). y: 8192 strided systems of 16384 rows,
+    longer than one workgroup holds (2048 strided rows): the long-system path
+    (hybrid segments reduced to one equation per chunk, the reduced systems
+    solved by a second batched solve, the segments read again for the
+    substitution; about 85 B per point moved). x8192: 8192 contiguous systems
+    of 8192 rows (256 lanes of 32 rows, two workgroups per CU)."""
     out = {}
     for name, sd, dims in (("x", 0, [16384, 8192, 1]), ("y", 1, [8192, 16384, 1]),
                            ("x8192", 0, [8192, 8192, 1])):

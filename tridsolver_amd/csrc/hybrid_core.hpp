@@ -596,8 +596,10 @@ __device__ __forceinline__ void long32_solve(int64_t base, int N, unsigned char 
   Ring ring{smem_raw + sizeof(REAL) * (size_t)T * (M + 1), (int)(3 * sizeof(REAL) * T), 0};
   const REAL *sb = b + base, *sc = c + base, *sd = d + base;
   // b, c, d in fetch order (lane t: rows t + T*r), from rows clamped into
-  // the system so that every load is unconditional; then identity rows past
-  // N and c of row N-1 set to their values
+  // the system so that every load is unconditional; the identity rows past N
+  // and c of row N-1 are set once the rows are in chunk order, where the
+  // lane's rows are row0 .. row0 + 31 (one row index, constant offsets,
+  // rather than 32 row indices held from the loads)
   REAL rb[M], rc[M], rd[M];
 #pragma unroll
   for (int r = 0; r < M; ++r) {
@@ -606,13 +608,6 @@ __device__ __forceinline__ void long32_solve(int64_t base, int N, unsigned char 
     rb[r] = sb[rr];
     rc[r] = sc[rr];
     rd[r] = sd[rr];
-  }
-#pragma unroll
-  for (int r = 0; r < M; ++r) {
-    const int row = t + T * r;
-    rb[r] = row < N ? rb[r] : REAL(1);
-    rc[r] = row < N - 1 ? rc[r] : REAL(0);
-    rd[r] = row < N ? rd[r] : REAL(0);
   }
   // transposes through the tile, hyb::to_chunk / to_fetch at P = T, M = 32:
   // fetch-order row t + Tr sits at slot skew<32>(t + Tr) = (t + t/32) +
@@ -631,6 +626,15 @@ __device__ __forceinline__ void long32_solve(int64_t base, int N, unsigned char 
   fetched();
   to_chunk(rc);
   to_chunk(rd);  // ends with a barrier: the tile is free
+  const int row0 = t * M;
+  if (row0 + M >= N) {
+#pragma unroll
+    for (int r = 0; r < M; ++r) {
+      rb[r] = row0 + r < N ? rb[r] : REAL(1);
+      rc[r] = row0 + r < N - 1 ? rc[r] : REAL(0);
+      rd[r] = row0 + r < N ? rd[r] : REAL(0);
+    }
+  }
   {
     // wave v moves the 256-byte windows k = v, v + T/64, ...; lane l the
     // dword l of window k (row k*E + l*E/64, E rows a window). Row 0's a is
@@ -658,7 +662,6 @@ __device__ __forceinline__ void long32_solve(int64_t base, int N, unsigned char 
       }
     }
   }
-  const int row0 = t * M;
   REAL *al_s = tile + t * (M + 1);  // this lane's rows, chunk order
   // a of the identity rows past N is 0: the lane clears its own slots, which
   // no load writes
@@ -685,9 +688,14 @@ __device__ __forceinline__ void long32_solve(int64_t base, int N, unsigned char 
   for (int r = 0; r < M; ++r) rd[r] = tile[fslot + (T + T / M) * r];
   __syncthreads();
   REAL *dst = (INC ? u : d) + base;
+  // the store rows formed here from a thread index the compiler cannot
+  // match with the loads' (an empty asm), so that the loads' row indices
+  // are not held through the solve
+  int ts = t;
+  asm volatile("" : "+v"(ts));
 #pragma unroll
   for (int r = 0; r < M; ++r) {
-    const int row = t + T * r;
+    const int row = ts + T * r;
     if (row < N) {
       if (INC) dst[row] += rd[r];
       else dst[row] = rd[r];
