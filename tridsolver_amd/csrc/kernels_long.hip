@@ -1,6 +1,7 @@
 // kernels_long.hip - batched solve of systems longer than one workgroup holds
-// (above 8192 rows at 16 rows per lane on 512 lanes), by the Thomas-PCR
-// hybrid with its reduced system solved outside the workgroups.
+// (contiguous: above 16384 rows; strided: above 2048 FP64 / 4096 FP32 rows),
+// by the Thomas-PCR hybrid with its reduced system solved outside the
+// workgroups.
 //
 // The hybrid (hybrid_core.hpp, kernels_solve.hip) cuts a system into chunks
 // of M rows, reduces every chunk to one equation in its last unknown y_k
