@@ -51,7 +51,9 @@ EXTERN_C size_t tridHipWorkspaceBytes(int fp64, int inc, int ndim, int solvedim,
  * 9 ragged mixed precision, 10 FP64 solution of FP32-stored systems, 11 the
  * same for a ragged batch, 12 systems longer than one workgroup holds: the
  * hybrid with its reduced system solved by a second batched solve, 13
- * contiguous systems of 4097-16384 rows, one per workgroup, with a in LDS),
+ * contiguous systems of 4097-16384 rows, one per workgroup, with a in LDS,
+ * 14 the adjoint of systems longer than its kernel holds: the transposed
+ * system solved by the batched solve),
  * and
  * for the hybrid its
  * rows per lane M, lanes per system P, systems per workgroup W and XCD
@@ -150,8 +152,13 @@ EXTERN_C tridStatus_t tridDmtsvStridedBatchMixedF32(const TridParams *ctx, const
  * entries a[0] and c[N-1] of every system get 0 (they are unused by the
  * solve). x must be the forward solution; it is read only for those three
  * gradients and may be NULL when all three pointers are NULL (the solve for
- * lam alone then moves the forward solve's bytes). Systems up to one
- * workgroup long (16384 rows FP32, 8192 FP64 on contiguous systems). */
+ * lam alone then moves the forward solve's bytes). Any length: systems
+ * longer than the adjoint kernel holds (contiguous, 4096 FP64 / 8192 FP32
+ * rows; strided, 8192) take three steps -- the transpose's off-diagonals
+ * written to the stream's workspace, A^T lam = w by the batched solve, then
+ * the gradients -- about 128 B per point with all three gradients at FP64.
+ * Under stream capture that workspace must already exist: run the same call
+ * once on the capturing stream beforehand. */
 EXTERN_C tridStatus_t tridDmtsvStridedBatchAdjoint(const TridParams *ctx, const double *a,
                                                    const double *b, const double *c,
                                                    const double *x, const double *w,

@@ -382,6 +382,61 @@ def test_adjoint_vs_oracle(dims, dtype):
         assert abs(np.dot(w, x) - np.dot(lam, d)) <= 50 * tol * np.abs(w).sum()
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("dims,sd", [([5000, 3], 0), ([20000, 2], 0), ([3, 9000], 1),
+                                     ([4, 3, 9000], 2), ([2, 17000, 2], 1)])
+def test_adjoint_long_vs_oracle(dims, sd, dtype):
+    """Systems longer than the adjoint kernel holds (FP64 contiguous above
+    4096 rows, any above 8192): the transpose's off-diagonals written out,
+    A^T lam = w by the batched solve at any length (the one-workgroup hybrid
+    or the long-system path), then the gradients. Against the oracle's
+    adjoint, with lam alone and with NULL x."""
+    rng = np.random.default_rng(sum(dims) + sd)
+    n = int(np.prod(dims))
+    a, b, c, d = H.diag_dominant(rng, n)
+    w = rng.standard_normal(n)
+    tol = 1e-12 if dtype == np.float64 else 1e-5
+    x = d.copy()
+    oracle.batch(a, b, c, x, ndim=len(dims), solvedim=sd, dims=dims)
+    ref = oracle.adjoint(a, b, c, x, w, ndim=len(dims), solvedim=sd, dims=dims)
+    t = [dev(v.astype(dtype)) for v in (a, b, c, x, w)]
+    out = api.adjoint_strided_batch(*t, solvedim=sd, dims=dims, grad_a=True, grad_b=True,
+                                    grad_c=True, sync=True)
+    algo = api.last_config()[0]
+    assert algo in (4, 14), algo
+    if dims[sd] > 8192 or (dtype == np.float64 and sd == 0):
+        assert algo == 14, algo
+    for got, want in zip(out, ref):
+        H.assert_norm_close(got.cpu().numpy(), want, tol=tol)
+    lam_only = api.adjoint_strided_batch(t[0], t[1], t[2], None, t[4], solvedim=sd, dims=dims,
+                                         sync=True)
+    assert np.array_equal(lam_only[0].cpu().numpy(), out[0].cpu().numpy())
+
+
+def test_adjoint_long_padded():
+    """The long-system adjoint on the interior of a halo-padded allocation:
+    the halo of lam and of every gradient is untouched."""
+    rng = np.random.default_rng(91)
+    dims = [3, 9000, 2]
+    n = int(np.prod(dims))
+    a, b, c, d = H.diag_dominant(rng, n)
+    w = rng.standard_normal(n)
+    x = d.copy()
+    oracle.batch(a, b, c, x, ndim=3, solvedim=1, dims=dims)
+    ref = oracle.adjoint(a, b, c, x, w, ndim=3, solvedim=1, dims=dims)
+    off, pads = H.halo_offset(dims), [v + 2 for v in dims]
+    ins = [dev(H.pad_halo(v, dims)) for v in (a, b, c, x, w)]
+    outs = [dev(np.full(int(np.prod(pads)), 7.0)) for _ in range(4)]
+    api.adjoint_strided_batch(*ins, solvedim=1, dims=dims, pads=pads, offset=off, lam=outs[0],
+                              grad_a=outs[1], grad_b=outs[2], grad_c=outs[3], sync=True)
+    assert api.last_config()[0] == 14
+    mask = H.pad_halo(np.ones(n, dtype=bool), dims)
+    for got, want in zip(outs, ref):
+        g = got.cpu().numpy()
+        H.assert_norm_close(g[mask], want, tol=1e-12)
+        assert np.all(g[~mask] == 7.0)
+
+
 def _fd_check(a, b, c, d, w, dims, sd, grads, rng, count=100, h=1e-6):
     """Central differences of L = sum(w * A^-1 d) on `count` random entries of
     each of a, b, c, d, evaluated on the GPU; L is summed over the perturbed

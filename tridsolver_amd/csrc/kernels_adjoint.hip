@@ -15,6 +15,10 @@
 // each requested gradient (72 B per point at FP64 with all three gradients).
 // When no coefficient gradient is requested (GRAD false) x is not read: the
 // solve for lam alone moves the forward solve's 40 B per point.
+// Systems longer than adjoint_kernel holds take three steps instead: the
+// transpose's off-diagonals written out (adj_transpose_kernel), A^T lam = w
+// solved by the batched solve, which takes any length, and the gradients
+// (adj_grad_kernel): about 128 B per point with all three gradients.
 
 #include <algorithm>
 #include <atomic>
@@ -136,7 +140,100 @@ int adj_max_threads(int M) {
   }
 }
 
+// ---- systems longer than the adjoint kernel holds -------------------------
+// Element e of the batch (0 <= e < nsys * N) in an order that keeps
+// neighbouring threads on neighbouring addresses: along the system for
+// contiguous systems, across x-neighbouring systems for strided ones. k is
+// the row inside the system.
+template <bool CONTIG>
+__device__ __forceinline__ int64_t adj_elem(const Geom &g, int64_t e, int64_t *k) {
+  int64_t i0, i1;
+  if (CONTIG) {
+    *k = e % g.N;
+    const int64_t s = e / g.N;
+    i0 = s % g.n0;
+    i1 = s / g.n0;
+  } else {
+    i0 = e % g.n0;
+    const int64_t r = e / g.n0;
+    *k = r % g.N;
+    i1 = r / g.N;
+  }
+  return i0 * g.st0 + i1 * g.st1 + *k * g.stride;
+}
+
+// The transpose's off-diagonals in the layout of the inputs, sub[n] = c[n-1]
+// and sup[n] = a[n+1] (0 where the row has none), and lam = w, the right-hand
+// side the batched solve then overwrites with the solution of A^T lam = w.
+template <typename REAL, bool CONTIG>
+__global__ __launch_bounds__(256) void adj_transpose_kernel(
+    Geom g, const REAL *__restrict__ a, const REAL *__restrict__ c, const REAL *__restrict__ w,
+    REAL *__restrict__ sub, REAL *__restrict__ sup, REAL *__restrict__ lam) {
+  const int64_t total = g.nsys() * g.N;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * 256) {
+    int64_t k;
+    const int64_t i = adj_elem<CONTIG>(g, e, &k);
+    sub[i] = k > 0 ? c[i - g.stride] : REAL(0);
+    sup[i] = k + 1 < g.N ? a[i + g.stride] : REAL(0);
+    lam[i] = w[i];
+  }
+}
+
+// dL/da[n] = -lam[n] x[n-1] (0 at n = 0), dL/db[n] = -lam[n] x[n],
+// dL/dc[n] = -lam[n] x[n+1] (0 at n = N-1), each where its output is given.
+template <typename REAL, bool CONTIG>
+__global__ __launch_bounds__(256) void adj_grad_kernel(Geom g, const REAL *__restrict__ x,
+                                                       const REAL *__restrict__ lam,
+                                                       REAL *__restrict__ ga,
+                                                       REAL *__restrict__ gb,
+                                                       REAL *__restrict__ gc) {
+  const int64_t total = g.nsys() * g.N;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * 256) {
+    int64_t k;
+    const int64_t i = adj_elem<CONTIG>(g, e, &k);
+    const REAL l = lam[i];
+    if (ga) ga[i] = k > 0 ? -l * x[i - g.stride] : REAL(0);
+    if (gb) gb[i] = -l * x[i];
+    if (gc) gc[i] = k + 1 < g.N ? -l * x[i + g.stride] : REAL(0);
+  }
+}
+
+unsigned adj_elem_blocks(const Geom &g) {
+  const int64_t total = g.nsys() * g.N;
+  return (unsigned)std::min<int64_t>((total + 255) / 256, int64_t(1) << 20);
+}
+
 }  // namespace
+
+size_t adjoint_long_span(const Geom &g) {
+  return (size_t)((g.n0 - 1) * g.st0 + (g.n1 - 1) * g.st1 + (g.N - 1) * g.stride + 1);
+}
+
+template <typename REAL>
+hipError_t launch_adjoint_transpose(const Geom &g, const REAL *a, const REAL *c, const REAL *w,
+                                    REAL *sub, REAL *sup, REAL *lam, hipStream_t st) {
+  if (g.stride == 1)
+    hipLaunchKernelGGL((adj_transpose_kernel<REAL, true>), dim3(adj_elem_blocks(g)), dim3(256),
+                       0, st, g, a, c, w, sub, sup, lam);
+  else
+    hipLaunchKernelGGL((adj_transpose_kernel<REAL, false>), dim3(adj_elem_blocks(g)),
+                       dim3(256), 0, st, g, a, c, w, sub, sup, lam);
+  return hipGetLastError();
+}
+
+template <typename REAL>
+hipError_t launch_adjoint_grads(const Geom &g, const REAL *x, const REAL *lam, REAL *ga,
+                                REAL *gb, REAL *gc, hipStream_t st) {
+  if (g.stride == 1)
+    hipLaunchKernelGGL((adj_grad_kernel<REAL, true>), dim3(adj_elem_blocks(g)), dim3(256), 0,
+                       st, g, x, lam, ga, gb, gc);
+  else
+    hipLaunchKernelGGL((adj_grad_kernel<REAL, false>), dim3(adj_elem_blocks(g)), dim3(256), 0,
+                       st, g, x, lam, ga, gb, gc);
+  return hipGetLastError();
+}
 
 template <typename REAL>
 bool adjoint_cfg_valid(const Geom &g, const HybridCfg &c) {
@@ -216,5 +313,14 @@ template hipError_t launch_adjoint<double>(const Geom &, const HybridCfg &, cons
                                            const double *, const double *, const double *,
                                            const double *, double *, double *, double *,
                                            double *, hipStream_t);
+#define TRID_ADJ_LONG_INST(REAL)                                                            \
+  template hipError_t launch_adjoint_transpose<REAL>(const Geom &, const REAL *, const REAL *, \
+                                                     const REAL *, REAL *, REAL *, REAL *,     \
+                                                     hipStream_t);                             \
+  template hipError_t launch_adjoint_grads<REAL>(const Geom &, const REAL *, const REAL *,    \
+                                                 REAL *, REAL *, REAL *, hipStream_t);
+TRID_ADJ_LONG_INST(float)
+TRID_ADJ_LONG_INST(double)
+#undef TRID_ADJ_LONG_INST
 
 }  // namespace trid

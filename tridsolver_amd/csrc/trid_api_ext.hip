@@ -108,7 +108,23 @@ tridStatus_t adjoint(const TridParams *ctx, const REAL *a, const REAL *b, const 
     cfg = forced;
     if (!adjoint_cfg_valid<REAL>(g, cfg)) return TRID_STATUS_INVALID_VALUE;
   } else if (!adjoint_tuned_config<REAL>(dev, g, ga || gb || gc, &cfg)) {
-    return TRID_STATUS_INVALID_VALUE;  // longer than one workgroup can hold
+    // longer than the adjoint kernel holds: the transpose's off-diagonals
+    // into the stream's role-2 workspace, A^T lam = w by the batched solve
+    // (which takes any length), then the gradients
+    const size_t span = adjoint_long_span(g);
+    void *ws = nullptr;
+    if ((s = get_workspace(dev, st, 2 * span * sizeof(REAL), &ws, 2)) != TRID_STATUS_SUCCESS)
+      return s;
+    REAL *sub = static_cast<REAL *>(ws), *sup = sub + span;
+    hipError_t e = launch_adjoint_transpose<REAL>(g, a, c, w, sub, sup, lam, st);
+    if (e != hipSuccess) return finish(ctx, st, e);
+    TridParams inner{};  // the solve's defaults, no synchronisation
+    if ((s = solve<REAL, false>(&inner, sub, b, sup, lam, nullptr, ndim, solvedim, dims, pads)) !=
+        TRID_STATUS_SUCCESS)
+      return s;
+    record_config(14, nullptr);
+    return finish(ctx, st, (ga || gb || gc) ? launch_adjoint_grads<REAL>(g, x, lam, ga, gb, gc, st)
+                                            : hipSuccess);
   }
   record_config(4, &cfg);
   return finish(ctx, st, for_each_grid_chunk(g, cfg.W, cfg.W * cfg.P,

@@ -72,7 +72,8 @@ tridStatus_t solve(const TridParams *ctx, const REAL *a, const REAL *b, const RE
                    const int *pads);
 hipStream_t current_stream();
 tridStatus_t check_device(int *dev);
-// role 0: the solve's / ragged plan's workspace; 1: partitioned-solve saves
+// role 0: the solve's / ragged plan's workspace; 1: partitioned-solve saves;
+// 2: the transposed coefficients of the long-system adjoint
 tridStatus_t get_workspace(int dev, hipStream_t st, size_t bytes, void **out, int role = 0);
 // Role-0 workspace bytes the per-lane path would take for this call (0 when
 // it runs a hybrid kernel), and the release of a destroyed stream's buffers.
@@ -128,6 +129,17 @@ template <typename REAL>
 hipError_t launch_adjoint(const Geom &g, const HybridCfg &cfg, const REAL *a, const REAL *b,
                           const REAL *c, const REAL *x, const REAL *w, REAL *lam, REAL *ga,
                           REAL *gb, REAL *gc, hipStream_t st);
+// Systems longer than the adjoint kernel holds: sub[n] = c[n-1], sup[n] =
+// a[n+1] and lam = w, in the inputs' layout (arrays of adjoint_long_span(g)
+// elements); after the batched solve of (sub, b, sup) into lam, the
+// requested gradients from lam and x.
+size_t adjoint_long_span(const Geom &g);
+template <typename REAL>
+hipError_t launch_adjoint_transpose(const Geom &g, const REAL *a, const REAL *c, const REAL *w,
+                                    REAL *sub, REAL *sup, REAL *lam, hipStream_t st);
+template <typename REAL>
+hipError_t launch_adjoint_grads(const Geom &g, const REAL *x, const REAL *lam, REAL *ga,
+                                REAL *gb, REAL *gc, hipStream_t st);
 
 // ---- ragged batch (kernels_ragged.hip) ------------------------------------
 // A ragged batch plans in ragged_workspace_bytes(S) bytes: this header (plan
