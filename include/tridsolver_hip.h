@@ -102,7 +102,10 @@ EXTERN_C tridStatus_t tridHipAutotuneMixed(int ndim, int solvedim, const int *di
 EXTERN_C tridStatus_t tridHipAutotuneMixedF32(int ndim, int solvedim, const int *dims,
                                               const int *pads, float *best_ms);
 /* Tunes trid{D,S}mtsvStridedBatchAdjoint for one geometry (fp64: 1 double /
- * 0 float; grads: 1 with the coefficient gradients, 0 for lambda only). */
+ * 0 float; grads: 1 with the coefficient gradients, 0 for lambda only).
+ * Systems longer than the adjoint kernel holds have nothing to tune (the
+ * transposed batched solve runs; its solve is tuned as tridHipAutotune
+ * tunes it): the call records that and sets *best_ms to 0. */
 EXTERN_C tridStatus_t tridHipAutotuneAdjoint(int fp64, int grads, int ndim, int solvedim,
                                              const int *dims, const int *pads, float *best_ms);
 /* Tunes the ragged kernel (rows per lane, workgroups per CU) on the caller's

@@ -413,6 +413,25 @@ def test_adjoint_long_vs_oracle(dims, sd, dtype):
     assert np.array_equal(lam_only[0].cpu().numpy(), out[0].cpu().numpy())
 
 
+def test_adjoint_long_autotune():
+    """Tuning a geometry the adjoint kernel does not hold records the
+    transposed path (nothing to time) and succeeds; the solve then runs it."""
+    dims = [9000, 3]
+    assert api.autotune_adjoint(torch.float64, 0, dims) == 0.0
+    rng = np.random.default_rng(92)
+    n = int(np.prod(dims))
+    a, b, c, d = H.diag_dominant(rng, n)
+    w = rng.standard_normal(n)
+    x = d.copy()
+    oracle.batch(a, b, c, x, ndim=2, solvedim=0, dims=dims)
+    ref = oracle.adjoint(a, b, c, x, w, ndim=2, solvedim=0, dims=dims)
+    out = api.adjoint_strided_batch(*(dev(v) for v in (a, b, c, x, w)), solvedim=0, dims=dims,
+                                    grad_b=True, sync=True)
+    assert api.last_config()[0] == 14
+    H.assert_norm_close(out[0].cpu().numpy(), ref[0], tol=1e-12)
+    H.assert_norm_close(out[2].cpu().numpy(), ref[2], tol=1e-12)
+
+
 def test_adjoint_long_padded():
     """The long-system adjoint on the interior of a halo-padded allocation:
     the halo of lam and of every gradient is untouched."""

@@ -489,6 +489,16 @@ bool mixed32_tuned_config(int dev, const Geom &g, int refine, HybridCfg *cfg) {
 template <typename REAL>
 tridStatus_t tune_adjoint(int dev, const Geom &g, bool grads, Choice *best) {
   (void)dev;
+  {
+    // no adjoint kernel shape holds these systems: the call takes the
+    // transposed batched solve, nothing to tune (algo 14, no time)
+    HybridCfg any;
+    if (!adjoint_default_cfg<REAL>(g, &any)) {
+      best->algo = 14;
+      best->ms = 0.0f;
+      return TRID_STATUS_SUCCESS;
+    }
+  }
   hipStream_t st = nullptr;
   if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess)
     return TRID_STATUS_EXECUTION_FAILED;
