@@ -53,8 +53,9 @@ EXTERN_C size_t tridHipWorkspaceBytes(int fp64, int inc, int ndim, int solvedim,
  * hybrid with its reduced system solved by a second batched solve, 13
  * contiguous systems of 4097-16384 rows, one per workgroup, with a in LDS,
  * 14 the adjoint of systems longer than its kernel holds: the transposed
- * system solved by the batched solve),
- * and
+ * system solved by the batched solve, 15 the mixed-precision call on
+ * systems no mixed kernel holds: the FP64 batched solve, 16 the same for
+ * FP32-stored systems: promoted to FP64, then the FP64 batched solve), and
  * for the hybrid its
  * rows per lane M, lanes per system P, systems per workgroup W and XCD
  * grouping G (consecutive system groups dealt to one XCD; 1 = plain order).
@@ -123,9 +124,11 @@ EXTERN_C tridStatus_t tridHipAutotuneClear(void);
  * of refinement steps (0..8; 2 reaches FP64 accuracy on diagonally dominant
  * systems). The matrix is factored and the sweeps run in FP32; the residual
  * d - A x and the accumulation x += dx are FP64. One pass over HBM: a, b, c,
- * d are read once and d is overwritten with the FP64 solution. Systems up to
- * 8192 rows along a contiguous dimension (16 rows per lane, 512 lanes) and
- * 4096 rows along a strided one (8 rows per lane). */
+ * d are read once and d is overwritten with the FP64 solution. The mixed
+ * kernels hold systems of up to 8192 rows along a contiguous dimension (16
+ * rows per lane, 512 lanes) and 4096 along a strided one (8 rows per lane);
+ * longer systems are solved by the FP64 batched solve (FP64 throughout, at
+ * least as accurate). */
 EXTERN_C tridStatus_t tridDmtsvStridedBatchMixed(const TridParams *ctx, const double *a,
                                                  const double *b, const double *c, double *d,
                                                  int ndim, int solvedim, const int *dims,
@@ -140,7 +143,12 @@ EXTERN_C tridStatus_t tridDmtsvStridedBatchMixed(const TridParams *ctx, const do
  * residual d - A x and x += dx in FP64, `refinements` (0..8) times; 1 reaches
  * about 1e-13 of the FP64 solution on diagonally dominant systems. One pass
  * over HBM: 16 B read and 8 B written per point, against 40 B for the FP64
- * solve of the same system stored in FP64. Systems up to 8192 rows. */
+ * solve of the same system stored in FP64. The kernels hold systems of up to
+ * 8192 rows; longer systems are promoted to FP64
+ * in the stream's workspace (24 B per point) and solved by the FP64 batched
+ * solve: the FP64 solution of the same promoted system, at about 88 B of
+ * traffic per point. Under stream capture that workspace must already exist:
+ * run the same call once on the capturing stream beforehand. */
 EXTERN_C tridStatus_t tridDmtsvStridedBatchMixedF32(const TridParams *ctx, const float *a,
                                                     const float *b, const float *c,
                                                     const float *d, double *x, int ndim,

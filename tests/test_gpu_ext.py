@@ -318,6 +318,59 @@ def test_mixed_f32_padded_layout():
         H.assert_norm_close(x[mask], want, np.float64, tol=1e-12)
 
 
+@pytest.mark.parametrize("dims,sd", [([9000, 3], 0), ([20000, 2], 0), ([3, 5000], 1),
+                                     ([4, 3, 4500], 2), ([2, 9000, 2], 1)])
+def test_mixed_long_systems(dims, sd):
+    """Systems no mixed kernel holds: the FP64-stored call (kernels up to
+    8192 contiguous / 4096 strided rows) runs the FP64 batched solve
+    (algorithm 15); the FP32-stored call (kernels up to 8192 rows) promotes
+    the batch and runs it too (algorithm 16). Within 1e-12 of the
+    long-double oracle of their system at every refinement count; the FP32
+    operands are left as they were."""
+    rng = np.random.default_rng(sum(dims) + 40)
+    n = int(np.prod(dims))
+    a, b, c, d = H.diag_dominant(rng, n)
+    xl = oracle.batch_ld(a, b, c, d, ndim=len(dims), solvedim=sd, dims=dims)
+    ta, tb, tc = (dev(t) for t in (a, b, c))
+    for refine in (0, 1):
+        td = dev(d)
+        api.mixed_strided_batch(ta, tb, tc, td, solvedim=sd, dims=dims, refinements=refine,
+                                sync=True)
+        assert api.last_config()[0] == 15
+        assert H.norm_rel_err(td.cpu().numpy(), xl) <= 1e-12
+    a32, b32, c32, d32 = _f32_case(rng, n)
+    xl32 = oracle.batch_ld(a32, b32, c32, d32, ndim=len(dims), solvedim=sd, dims=dims)
+    t32 = [dev(t) for t in (a32, b32, c32, d32)]
+    promoted = dims[sd] > 8192
+    for refine in (1, 2) if not promoted else (0, 2):
+        x = api.mixed_f32_strided_batch(*t32, solvedim=sd, dims=dims, refinements=refine,
+                                        sync=True)
+        assert api.last_config()[0] == (16 if promoted else 10)
+        assert H.norm_rel_err(x.cpu().numpy(), xl32) <= 1e-12
+    assert np.array_equal(t32[3].cpu().numpy(), d32)
+    if promoted:
+        assert api.autotune_mixed_f32(sd, dims) == 0.0  # nothing to tune: recorded
+
+
+def test_mixed_f32_long_padded():
+    """The promoted long-system path on the interior of a halo-padded
+    allocation: x's halo is untouched."""
+    rng = np.random.default_rng(93)
+    dims = [3, 9000, 2]
+    n = int(np.prod(dims))
+    a, b, c, d = _f32_case(rng, n)
+    xl = oracle.batch_ld(a, b, c, d, ndim=3, solvedim=1, dims=dims)
+    off, pads = H.halo_offset(dims), [v + 2 for v in dims]
+    t = [dev(H.pad_halo(v, dims)) for v in (a, b, c, d)]
+    x = dev(np.full(int(np.prod(pads)), 7.0))
+    api.mixed_f32_strided_batch(*t, x, solvedim=1, dims=dims, pads=pads, offset=off, sync=True)
+    assert api.last_config()[0] == 16
+    g = x.cpu().numpy()
+    mask = H.pad_halo(np.ones(n, dtype=bool), dims)
+    assert np.all(g[~mask] == 7.0)
+    assert H.norm_rel_err(g[mask], xl) <= 1e-12
+
+
 def test_mixed_f32_rejects_bad_arguments():
     rng = np.random.default_rng(92)
     dims = [16, 8, 4]

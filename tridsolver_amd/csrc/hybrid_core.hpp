@@ -57,6 +57,27 @@ struct Lane {
   bool active;     // the lane belongs to an existing system
 };
 
+// Element e of the batch (0 <= e < nsys * N), for elementwise kernels, in an
+// order that keeps neighbouring threads on neighbouring addresses: along the
+// system for contiguous systems, across x-neighbouring systems for strided
+// ones. *k is the row inside the system; the result the element's index.
+template <bool CONTIG>
+__device__ __forceinline__ int64_t batch_elem(const Geom &g, int64_t e, int64_t *k) {
+  int64_t i0, i1;
+  if (CONTIG) {
+    *k = e % g.N;
+    const int64_t s = e / g.N;
+    i0 = s % g.n0;
+    i1 = s / g.n0;
+  } else {
+    i0 = e % g.n0;
+    const int64_t r = e / g.n0;
+    *k = r % g.N;
+    i1 = r / g.N;
+  }
+  return i0 * g.st0 + i1 * g.st1 + *k * g.stride;
+}
+
 // Lane of a batched (Geom) launch: system groups of W x-neighbours (strided)
 // or W rows (contiguous), P lanes per system.
 template <bool CONTIG>

@@ -141,27 +141,6 @@ int adj_max_threads(int M) {
 }
 
 // ---- systems longer than the adjoint kernel holds -------------------------
-// Element e of the batch (0 <= e < nsys * N) in an order that keeps
-// neighbouring threads on neighbouring addresses: along the system for
-// contiguous systems, across x-neighbouring systems for strided ones. k is
-// the row inside the system.
-template <bool CONTIG>
-__device__ __forceinline__ int64_t adj_elem(const Geom &g, int64_t e, int64_t *k) {
-  int64_t i0, i1;
-  if (CONTIG) {
-    *k = e % g.N;
-    const int64_t s = e / g.N;
-    i0 = s % g.n0;
-    i1 = s / g.n0;
-  } else {
-    i0 = e % g.n0;
-    const int64_t r = e / g.n0;
-    *k = r % g.N;
-    i1 = r / g.N;
-  }
-  return i0 * g.st0 + i1 * g.st1 + *k * g.stride;
-}
-
 // The transpose's off-diagonals in the layout of the inputs, sub[n] = c[n-1]
 // and sup[n] = a[n+1] (0 where the row has none), and lam = w, the right-hand
 // side the batched solve then overwrites with the solution of A^T lam = w.
@@ -173,7 +152,7 @@ __global__ __launch_bounds__(256) void adj_transpose_kernel(
   for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total;
        e += (int64_t)gridDim.x * 256) {
     int64_t k;
-    const int64_t i = adj_elem<CONTIG>(g, e, &k);
+    const int64_t i = hyb::batch_elem<CONTIG>(g, e, &k);
     sub[i] = k > 0 ? c[i - g.stride] : REAL(0);
     sup[i] = k + 1 < g.N ? a[i + g.stride] : REAL(0);
     lam[i] = w[i];
@@ -192,7 +171,7 @@ __global__ __launch_bounds__(256) void adj_grad_kernel(Geom g, const REAL *__res
   for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total;
        e += (int64_t)gridDim.x * 256) {
     int64_t k;
-    const int64_t i = adj_elem<CONTIG>(g, e, &k);
+    const int64_t i = hyb::batch_elem<CONTIG>(g, e, &k);
     const REAL l = lam[i];
     if (ga) ga[i] = k > 0 ? -l * x[i - g.stride] : REAL(0);
     if (gb) gb[i] = -l * x[i];
@@ -200,25 +179,16 @@ __global__ __launch_bounds__(256) void adj_grad_kernel(Geom g, const REAL *__res
   }
 }
 
-unsigned adj_elem_blocks(const Geom &g) {
-  const int64_t total = g.nsys() * g.N;
-  return (unsigned)std::min<int64_t>((total + 255) / 256, int64_t(1) << 20);
-}
-
 }  // namespace
-
-size_t adjoint_long_span(const Geom &g) {
-  return (size_t)((g.n0 - 1) * g.st0 + (g.n1 - 1) * g.st1 + (g.N - 1) * g.stride + 1);
-}
 
 template <typename REAL>
 hipError_t launch_adjoint_transpose(const Geom &g, const REAL *a, const REAL *c, const REAL *w,
                                     REAL *sub, REAL *sup, REAL *lam, hipStream_t st) {
   if (g.stride == 1)
-    hipLaunchKernelGGL((adj_transpose_kernel<REAL, true>), dim3(adj_elem_blocks(g)), dim3(256),
+    hipLaunchKernelGGL((adj_transpose_kernel<REAL, true>), dim3(batch_elem_blocks(g)), dim3(256),
                        0, st, g, a, c, w, sub, sup, lam);
   else
-    hipLaunchKernelGGL((adj_transpose_kernel<REAL, false>), dim3(adj_elem_blocks(g)),
+    hipLaunchKernelGGL((adj_transpose_kernel<REAL, false>), dim3(batch_elem_blocks(g)),
                        dim3(256), 0, st, g, a, c, w, sub, sup, lam);
   return hipGetLastError();
 }
@@ -227,10 +197,10 @@ template <typename REAL>
 hipError_t launch_adjoint_grads(const Geom &g, const REAL *x, const REAL *lam, REAL *ga,
                                 REAL *gb, REAL *gc, hipStream_t st) {
   if (g.stride == 1)
-    hipLaunchKernelGGL((adj_grad_kernel<REAL, true>), dim3(adj_elem_blocks(g)), dim3(256), 0,
+    hipLaunchKernelGGL((adj_grad_kernel<REAL, true>), dim3(batch_elem_blocks(g)), dim3(256), 0,
                        st, g, x, lam, ga, gb, gc);
   else
-    hipLaunchKernelGGL((adj_grad_kernel<REAL, false>), dim3(adj_elem_blocks(g)), dim3(256), 0,
+    hipLaunchKernelGGL((adj_grad_kernel<REAL, false>), dim3(batch_elem_blocks(g)), dim3(256), 0,
                        st, g, x, lam, ga, gb, gc);
   return hipGetLastError();
 }

@@ -30,6 +30,9 @@
 // More than one refinement keeps x in FP64 and d through the loop (the
 // general loop of mixed_core.hpp): only the shapes whose general kernel fits
 // the register file without spilling offer it (general_ok).
+// Systems none of them holds (above 8192 rows) are promoted to FP64 (mixed32_promote_kernel) and solved by the
+// FP64 batched solve: the FP64 solution of the same promoted system, at 24 B
+// of workspace and about 88 B of traffic per point.
 
 #include <algorithm>
 #include <atomic>
@@ -462,6 +465,39 @@ bool mixed32_default_cfg(const Geom &g, int refine, HybridCfg *cfg) {
     if (mixed32_cfg_valid(g, c, refine)) { *cfg = c; return true; }
   }
   return false;
+}
+
+namespace {
+
+template <bool CONTIG>
+__global__ __launch_bounds__(256) void mixed32_promote_kernel(
+    Geom g, const float *__restrict__ a, const float *__restrict__ b,
+    const float *__restrict__ c, const float *__restrict__ d, double *__restrict__ A,
+    double *__restrict__ B, double *__restrict__ C, double *__restrict__ x) {
+  const int64_t total = g.nsys() * g.N;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * 256) {
+    int64_t k;
+    const int64_t i = hyb::batch_elem<CONTIG>(g, e, &k);
+    A[i] = (double)a[i];
+    B[i] = (double)b[i];
+    C[i] = (double)c[i];
+    x[i] = (double)d[i];
+  }
+}
+
+}  // namespace
+
+hipError_t launch_mixed32_promote(const Geom &g, const float *a, const float *b, const float *c,
+                                  const float *d, double *A, double *B, double *C, double *x,
+                                  hipStream_t st) {
+  if (g.stride == 1)
+    hipLaunchKernelGGL(mixed32_promote_kernel<true>, dim3(batch_elem_blocks(g)), dim3(256), 0,
+                       st, g, a, b, c, d, A, B, C, x);
+  else
+    hipLaunchKernelGGL(mixed32_promote_kernel<false>, dim3(batch_elem_blocks(g)), dim3(256), 0,
+                       st, g, a, b, c, d, A, B, C, x);
+  return hipGetLastError();
 }
 
 hipError_t launch_mixed32(const Geom &g, const HybridCfg &cfg, int refine, const float *a,

@@ -111,7 +111,7 @@ tridStatus_t adjoint(const TridParams *ctx, const REAL *a, const REAL *b, const 
     // longer than the adjoint kernel holds: the transpose's off-diagonals
     // into the stream's role-2 workspace, A^T lam = w by the batched solve
     // (which takes any length), then the gradients
-    const size_t span = adjoint_long_span(g);
+    const size_t span = batch_span(g);
     void *ws = nullptr;
     if ((s = get_workspace(dev, st, 2 * span * sizeof(REAL), &ws, 2)) != TRID_STATUS_SUCCESS)
       return s;
@@ -190,7 +190,14 @@ EXTERN_C tridStatus_t tridDmtsvStridedBatchMixed(const TridParams *ctx, const do
     cfg = forced;
     if (!mixed_cfg_valid(g, cfg)) return TRID_STATUS_INVALID_VALUE;
   } else if (!mixed_tuned_config(dev, g, &cfg)) {
-    return TRID_STATUS_INVALID_VALUE;
+    // systems no mixed kernel holds: the FP64 batched solve (any length, at
+    // least the accuracy the refinements reach)
+    TridParams inner{};  // the solve's defaults, no synchronisation
+    if ((s = solve<double, false>(&inner, a, b, c, d, nullptr, ndim, solvedim, dims, pads)) !=
+        TRID_STATUS_SUCCESS)
+      return s;
+    record_config(15, nullptr);
+    return finish(ctx, st, hipSuccess);
   }
   record_config(2, &cfg);
   return finish(ctx, st, for_each_grid_chunk(g, cfg.W, cfg.W * cfg.P,
@@ -220,7 +227,22 @@ EXTERN_C tridStatus_t tridDmtsvStridedBatchMixedF32(const TridParams *ctx, const
     cfg = forced;
     if (!mixed32_cfg_valid(g, cfg, refinements)) return TRID_STATUS_INVALID_VALUE;
   } else if (!mixed32_tuned_config(dev, g, refinements, &cfg)) {
-    return TRID_STATUS_INVALID_VALUE;
+    // systems no such kernel holds: a, b, c promoted into the stream's role-2
+    // workspace and d into x, then the FP64 batched solve of the promoted
+    // system (any length) in x
+    const size_t span = batch_span(g);
+    void *ws = nullptr;
+    if ((s = get_workspace(dev, st, 3 * span * sizeof(double), &ws, 2)) != TRID_STATUS_SUCCESS)
+      return s;
+    double *A = static_cast<double *>(ws), *B = A + span, *C = B + span;
+    const hipError_t e = launch_mixed32_promote(g, a, b, c, d, A, B, C, x, st);
+    if (e != hipSuccess) return finish(ctx, st, e);
+    TridParams inner{};  // the solve's defaults, no synchronisation
+    if ((s = solve<double, false>(&inner, A, B, C, x, nullptr, ndim, solvedim, dims, pads)) !=
+        TRID_STATUS_SUCCESS)
+      return s;
+    record_config(16, nullptr);
+    return finish(ctx, st, hipSuccess);
   }
   record_config(10, &cfg);
   return finish(ctx, st, for_each_grid_chunk(g, cfg.W, cfg.W * cfg.P,

@@ -49,6 +49,18 @@ struct HybridCfg {
 
 enum class Algo : int { Auto = 0, Lane = 1, Hybrid = 3 };
 
+// Elements from the first element of a batch to its last, pads included
+// (the size of an array in the batch's layout), and the grid of the
+// grid-stride elementwise kernels over its nsys * N elements (256 threads).
+inline size_t batch_span(const Geom &g) {
+  return (size_t)((g.n0 - 1) * g.st0 + (g.n1 - 1) * g.st1 + (g.N - 1) * g.stride + 1);
+}
+inline unsigned batch_elem_blocks(const Geom &g) {
+  const int64_t total = g.nsys() * g.N;
+  const int64_t blocks = (total + 255) / 256;
+  return (unsigned)(blocks < (int64_t(1) << 20) ? blocks : (int64_t(1) << 20));
+}
+
 // ---- launchers (kernels_solve.hip) --------------------------------------
 // Per-lane Thomas, any length. cws: N * nsys reals of workspace; dws: same
 // size, only needed for Inc (the plain solve keeps d' in d itself).
@@ -73,7 +85,8 @@ tridStatus_t solve(const TridParams *ctx, const REAL *a, const REAL *b, const RE
 hipStream_t current_stream();
 tridStatus_t check_device(int *dev);
 // role 0: the solve's / ragged plan's workspace; 1: partitioned-solve saves;
-// 2: the transposed coefficients of the long-system adjoint
+// 2: the transposed coefficients of the long-system adjoint, the promoted
+// coefficients of the long-system FP32-stored solve
 tridStatus_t get_workspace(int dev, hipStream_t st, size_t bytes, void **out, int role = 0);
 // Role-0 workspace bytes the per-lane path would take for this call (0 when
 // it runs a hybrid kernel), and the release of a destroyed stream's buffers.
@@ -119,6 +132,12 @@ bool mixed32_default_cfg(const Geom &g, int refine, HybridCfg *cfg);
 hipError_t launch_mixed32(const Geom &g, const HybridCfg &cfg, int refine, const float *a,
                           const float *b, const float *c, const float *d, double *x,
                           hipStream_t st);
+// Systems no such kernel holds: a, b, c promoted to FP64 arrays A, B, C of
+// batch_span(g) elements in the batch's layout, and d to x, which the FP64
+// batched solve of (A, B, C) then overwrites with the solution.
+hipError_t launch_mixed32_promote(const Geom &g, const float *a, const float *b, const float *c,
+                                  const float *d, double *A, double *B, double *C, double *x,
+                                  hipStream_t st);
 
 // ---- adjoint (kernels_adjoint.hip) ----------------------------------------
 template <typename REAL>
@@ -130,10 +149,9 @@ hipError_t launch_adjoint(const Geom &g, const HybridCfg &cfg, const REAL *a, co
                           const REAL *c, const REAL *x, const REAL *w, REAL *lam, REAL *ga,
                           REAL *gb, REAL *gc, hipStream_t st);
 // Systems longer than the adjoint kernel holds: sub[n] = c[n-1], sup[n] =
-// a[n+1] and lam = w, in the inputs' layout (arrays of adjoint_long_span(g)
+// a[n+1] and lam = w, in the inputs' layout (arrays of batch_span(g)
 // elements); after the batched solve of (sub, b, sup) into lam, the
 // requested gradients from lam and x.
-size_t adjoint_long_span(const Geom &g);
 template <typename REAL>
 hipError_t launch_adjoint_transpose(const Geom &g, const REAL *a, const REAL *c, const REAL *w,
                                     REAL *sub, REAL *sup, REAL *lam, hipStream_t st);

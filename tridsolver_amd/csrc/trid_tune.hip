@@ -290,6 +290,16 @@ tridStatus_t tune_solve(int dev, const Geom &g, Choice *best) {
 
 tridStatus_t tune_mixed(int dev, const Geom &g, Choice *best) {
   (void)dev;
+  {
+    // no mixed kernel shape holds these systems: the call runs the FP64
+    // batched solve, nothing to tune here (algo 15, no time)
+    HybridCfg any;
+    if (!mixed_default_cfg(g, &any)) {
+      best->algo = 15;
+      best->ms = 0.0f;
+      return TRID_STATUS_SUCCESS;
+    }
+  }
   hipStream_t st = nullptr;
   if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess)
     return TRID_STATUS_EXECUTION_FAILED;
@@ -344,6 +354,16 @@ tridStatus_t tune_mixed(int dev, const Geom &g, Choice *best) {
 // `refine` are skipped (mixed32_cfg_valid).
 tridStatus_t tune_mixed32(int dev, const Geom &g, int refine, Choice *best) {
   (void)dev;
+  {
+    // no shape holds these systems: the call promotes them and runs the FP64
+    // batched solve, nothing to tune here (algo 16, no time)
+    HybridCfg any;
+    if (!mixed32_default_cfg(g, refine, &any)) {
+      best->algo = 16;
+      best->ms = 0.0f;
+      return TRID_STATUS_SUCCESS;
+    }
+  }
   hipStream_t st = nullptr;
   if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess)
     return TRID_STATUS_EXECUTION_FAILED;
