@@ -61,19 +61,34 @@ struct Lane {
 // order that keeps neighbouring threads on neighbouring addresses: along the
 // system for contiguous systems, across x-neighbouring systems for strided
 // ones. *k is the row inside the system; the result the element's index.
+// (32-bit divisions while e < 2^32, which covers every batch of fewer than
+// 2^32 elements; 64-bit ones beyond.)
+template <typename U, bool CONTIG>
+__device__ __forceinline__ void batch_split(U e, U N, U n0, U *k, U *i0, U *i1) {
+  if (CONTIG) {
+    const U s = e / N;
+    *k = e - s * N;
+    *i1 = s / n0;
+    *i0 = s - *i1 * n0;
+  } else {
+    const U r = e / n0;
+    *i0 = e - r * n0;
+    *i1 = r / N;
+    *k = r - *i1 * N;
+  }
+}
+
 template <bool CONTIG>
 __device__ __forceinline__ int64_t batch_elem(const Geom &g, int64_t e, int64_t *k) {
   int64_t i0, i1;
-  if (CONTIG) {
-    *k = e % g.N;
-    const int64_t s = e / g.N;
-    i0 = s % g.n0;
-    i1 = s / g.n0;
+  if ((uint64_t)e >> 32 == 0) {
+    uint32_t kk, j0, j1;
+    batch_split<uint32_t, CONTIG>((uint32_t)e, (uint32_t)g.N, (uint32_t)g.n0, &kk, &j0, &j1);
+    *k = kk; i0 = j0; i1 = j1;
   } else {
-    i0 = e % g.n0;
-    const int64_t r = e / g.n0;
-    *k = r % g.N;
-    i1 = r / g.N;
+    uint64_t kk, j0, j1;
+    batch_split<uint64_t, CONTIG>((uint64_t)e, (uint64_t)g.N, (uint64_t)g.n0, &kk, &j0, &j1);
+    *k = (int64_t)kk; i0 = (int64_t)j0; i1 = (int64_t)j1;
   }
   return i0 * g.st0 + i1 * g.st1 + *k * g.stride;
 }
