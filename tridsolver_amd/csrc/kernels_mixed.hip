@@ -103,7 +103,8 @@ __global__ __launch_bounds__(MIXED_THREADS) void mixed_tile_kernel(
   double *tC = tA + (size_t)L.T * (M + 1);
   hyb::Ring ring{smem_raw + 2 * sizeof(double) * (size_t)L.T * (M + 1),
                  (int)(2 * sizeof(double) * L.T), 0};
-  mixed_contig_lds<M, MIXED_MAX_STEPS>(L, ring, a, b, c, d, tA, tC, refine, [] {});
+  mixed_contig_lds<M, MIXED_MAX_STEPS>(L, ring, a, b, c, d, tA, tC, refine, [] {},
+                                       [&] { return L.base; });
 }
 
 typedef __attribute__((address_space(3))) void lds_void;

@@ -619,6 +619,10 @@ __global__ __launch_bounds__(T, 2) void rg_solve_mixed_lds(
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ int s_next;
   __shared__ RgPlanView s_view;
+  // each thread's system offset of the current item, read again for the
+  // stores: with the next item decoded during the solve, holding both in
+  // registers across it would spill
+  __shared__ int64_t s_base[T];
   rg_view_load(s_view, plan, threadIdx.x);
   const int t = threadIdx.x;
   int *counter = LONG ? &plan->work : &plan->work_short;
@@ -636,6 +640,7 @@ __global__ __launch_bounds__(T, 2) void rg_solve_mixed_lds(
     if (item >= total) break;
     // claim the following item now; its id is read after the first transpose
     if (t == 0) s_next = lo + atomicAdd(counter, 1);
+    s_base[t] = cur.base;
     const hyb::Lane L = rg_lane(cur, t, T);
     double *tA = tiles + (size_t)L.w * L.P * (M + 1);
     double *tC = tA + (size_t)T * (M + 1);
@@ -645,7 +650,7 @@ __global__ __launch_bounds__(T, 2) void rg_solve_mixed_lds(
     mixed_contig_lds<M, T == 512 ? 9 : 8>(L, ring, a, b, c, d, tA, tC, refine, [&] {
       nxt = s_next;
       if (nxt < total) next = rg_meta(s_view, list, off, nxt, t, T);
-    });
+    }, [&] { return s_base[t]; });
     item = nxt;
     cur = next;
   }

@@ -243,14 +243,17 @@ __device__ __forceinline__ void mixed_chunk(const hyb::Lane &L, hyb::Ring &ring,
 // Rows are 32-bit here (p + P*r < P*M <= 8192) and each array is addressed
 // from one per-lane pointer, so no 64-bit row index per row stays live (the
 // shared fetch's would, through to the store, and spill at this register
-// pressure).
-template <int M, int STEPS, typename OnFetched>
+// pressure). `store_base()` gives L.base again for the stores: a caller
+// whose register file is full returns it from an LDS slot, so that no
+// 64-bit base stays live across the solve.
+template <int M, int STEPS, typename OnFetched, typename StoreBase>
 __device__ __forceinline__ void mixed_contig_lds(const hyb::Lane &L, hyb::Ring &ring,
                                                  const double *__restrict__ a,
                                                  const double *__restrict__ b,
                                                  const double *__restrict__ c,
                                                  double *__restrict__ d, double *tA, double *tC,
-                                                 int refine, OnFetched fetched) {
+                                                 int refine, OnFetched fetched,
+                                                 StoreBase store_base) {
   const int P = L.P, p = L.p, N = L.active ? (int)L.N : 0;
   const int64_t lb = L.base + p;
   double A[M], B[M], Cc[M], D[M];
@@ -283,7 +286,7 @@ __device__ __forceinline__ void mixed_contig_lds(const hyb::Lane &L, hyb::Ring &
   hyb::to_fetch<M, true>(L, X, tA);
   // the store base is recomputed behind an empty asm, so that the load
   // addresses are not kept live across the solve
-  int64_t sb = lb;
+  int64_t sb = store_base() + p;
   asm volatile("" : "+v"(sb));
   double *qd = d + sb;
 #pragma unroll
