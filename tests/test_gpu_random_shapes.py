@@ -1,5 +1,6 @@
 """Seeded random sweep of the C API's input space against the oracle
-(plain and Inc solves, both mixed-precision solves, ragged batches).
+(plain and Inc solves, short and long systems, both mixed-precision solves,
+ragged batches).
 
 Each case draws ndim 1-3, extents 1..70 (one dimension sometimes up to 600),
 pads of up to 3 extra elements per dimension, a base offset into the
@@ -72,6 +73,53 @@ def test_random_shape(case):
         H.assert_norm_close(got_d, od, dtype)
         H.ref_allclose(od, got_d, dtype)
         # outside the solved region nothing changes (the oracle leaves it as is)
+        solved = np.zeros(total, dtype=bool)
+        for idx in H.system_slices(ndim, sd, dims, pads, offset):
+            solved[idx] = True
+        assert np.array_equal(got_d[~solved], d[~solved]), what + ": wrote outside the systems"
+
+
+def draw_long(rng):
+    """A random layout whose solve dimension is long: 4097..40000 rows
+    (log-uniform), the other extents 1..4, so that contiguous systems take
+    the one-workgroup solves of up to 16384 rows or the long-system path, and
+    strided ones the long-system path."""
+    ndim = int(rng.integers(1, 4))
+    sd = int(rng.integers(0, ndim))
+    dims = [int(rng.integers(1, 5)) for _ in range(ndim)]
+    dims[sd] = int(np.round(np.exp(rng.uniform(np.log(4097), np.log(40000)))))
+    pads = [d + int(rng.integers(0, 4)) for d in dims]
+    dtype = np.float64 if rng.random() < 0.6 else np.float32
+    inc = bool(rng.random() < 0.4)
+    offset = int(rng.integers(0, 5))
+    return ndim, dims, pads, sd, dtype, inc, offset
+
+
+@pytest.mark.parametrize("case", range(24))
+def test_random_long_shape(case):
+    """Plain and Inc solves of long systems on random layouts: as
+    test_random_shape, against the oracle; the configuration the call ran is
+    named in the failure message."""
+    rng = np.random.default_rng(5000 + case)
+    ndim, dims, pads, sd, dtype, inc, offset = draw_long(rng)
+    total = offset + int(np.prod(pads)) + 3
+    a, b, c, d = H.diag_dominant(rng, total, dtype)
+    u = rng.uniform(-1, 1, total).astype(dtype) if inc else None
+    ta, tb, tc, td = (torch.from_numpy(t.copy()).cuda() for t in (a, b, c, d))
+    tu = torch.from_numpy(u.copy()).cuda() if inc else None
+    api.trid_strided_batch(ta, tb, tc, td, solvedim=sd, dims=dims, pads=pads, u=tu,
+                           offset=offset, sync=True)
+    od, ou = d.copy(), (u.copy() if inc else None)
+    oracle.batch(a, b, c, od, ou, ndim=ndim, solvedim=sd, dims=dims, pads=pads, inc=inc,
+                 offset=offset)
+    what = "case %d: dims %s pads %s sd %d %s inc %s offset %d config %s" % (
+        case, dims, pads, sd, np.dtype(dtype).name, inc, offset, api.last_config())
+    got_d = td.cpu().numpy()
+    if inc:
+        assert np.array_equal(got_d, d), what + ": Inc wrote d"
+        H.assert_norm_close(tu.cpu().numpy() - u, ou - u, dtype)
+    else:
+        H.assert_norm_close(got_d, od, dtype)
         solved = np.zeros(total, dtype=bool)
         for idx in H.system_slices(ndim, sd, dims, pads, offset):
             solved[idx] = True
