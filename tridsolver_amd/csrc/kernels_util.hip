@@ -12,7 +12,8 @@
 //                        a by global->LDS loads into a [row][system] tile, b, c,
 //                        d into registers, d written back), with one add per
 //                        value in place of the solve: the bandwidth that access
-//                        pattern allows, against which the solve's is judged.
+//                        pattern allows, the ceiling the solve's rate is
+//                        compared with.
 #include "../../include/tridsolver_hip.h"
 #include <algorithm>
 
