@@ -35,7 +35,7 @@ struct AdjBounds {
   static constexpr int threads = sizeof(REAL) * M >= 64 ? 512 : 1024;
 };
 
-template <typename REAL, int M, bool CONTIG, bool GRAD>
+template <typename REAL, int M, bool CONTIG, bool GRAD, bool NT>
 __global__ __launch_bounds__((AdjBounds<REAL, M>::threads)) void adjoint_kernel(
     Geom g, int P, int W, int G, const REAL *__restrict__ a, const REAL *__restrict__ b,
     const REAL *__restrict__ c, const REAL *__restrict__ x, const REAL *__restrict__ w,
@@ -48,11 +48,11 @@ __global__ __launch_bounds__((AdjBounds<REAL, M>::threads)) void adjoint_kernel(
   hyb::Ring ring{smem_raw + tile_bytes, (int)(3 * sizeof(REAL) * L.T), 0};
 
   REAL ra[M], rb[M], rc[M], rd[M], rx[M];
-  hyb::fetch<M, CONTIG>(L, c, ra, REAL(0), -1, 0, L.N);  // sub of A^T: c[n-1]
-  hyb::fetch<M, CONTIG>(L, b, rb, REAL(1), 0, 0, L.N);
-  hyb::fetch<M, CONTIG>(L, a, rc, REAL(0), +1, 1, L.N);  // sup of A^T: a[n+1]
-  hyb::fetch<M, CONTIG>(L, w, rd, REAL(0), 0, 0, L.N);
-  if (GRAD) hyb::fetch<M, CONTIG>(L, x, rx, REAL(0), 0, 0, L.N);
+  hyb::fetch<M, CONTIG, NT>(L, c, ra, REAL(0), -1, 0, L.N);  // sub of A^T: c[n-1]
+  hyb::fetch<M, CONTIG, NT>(L, b, rb, REAL(1), 0, 0, L.N);
+  hyb::fetch<M, CONTIG, NT>(L, a, rc, REAL(0), +1, 1, L.N);  // sup of A^T: a[n+1]
+  hyb::fetch<M, CONTIG, NT>(L, w, rd, REAL(0), 0, 0, L.N);
+  if (GRAD) hyb::fetch<M, CONTIG, NT>(L, x, rx, REAL(0), 0, 0, L.N);
   hyb::to_chunk<M, CONTIG>(L, ra, tile);
   hyb::to_chunk<M, CONTIG>(L, rb, tile);
   hyb::to_chunk<M, CONTIG>(L, rc, tile);
@@ -62,7 +62,7 @@ __global__ __launch_bounds__((AdjBounds<REAL, M>::threads)) void adjoint_kernel(
   hyb::solve_chunk<REAL, M, CONTIG>(L, ring, ra, rb, rc, rd, l);
   if (!GRAD) {
     hyb::to_fetch<M, CONTIG>(L, l, tile);
-    hyb::store<M, CONTIG>(L, lam, l);
+    hyb::store<M, CONTIG, NT>(L, lam, l);
     return;
   }
 
@@ -86,14 +86,14 @@ __global__ __launch_bounds__((AdjBounds<REAL, M>::threads)) void adjoint_kernel(
       v[r] = (row0 + r == 0) ? REAL(0) : -l[r] * xl;
     }
     hyb::to_fetch<M, CONTIG>(L, v, tile);
-    hyb::store<M, CONTIG>(L, ga, v);
+    hyb::store<M, CONTIG, NT>(L, ga, v);
   }
   if (gb) {
     REAL v[M];
 #pragma unroll
     for (int r = 0; r < M; ++r) v[r] = -l[r] * rx[r];
     hyb::to_fetch<M, CONTIG>(L, v, tile);
-    hyb::store<M, CONTIG>(L, gb, v);
+    hyb::store<M, CONTIG, NT>(L, gb, v);
   }
   if (gc) {
     REAL v[M];
@@ -103,20 +103,23 @@ __global__ __launch_bounds__((AdjBounds<REAL, M>::threads)) void adjoint_kernel(
       v[r] = (row0 + r >= L.N - 1) ? REAL(0) : -l[r] * xr;
     }
     hyb::to_fetch<M, CONTIG>(L, v, tile);
-    hyb::store<M, CONTIG>(L, gc, v);
+    hyb::store<M, CONTIG, NT>(L, gc, v);
   }
   hyb::to_fetch<M, CONTIG>(L, l, tile);
-  hyb::store<M, CONTIG>(L, lam, l);
+  hyb::store<M, CONTIG, NT>(L, lam, l);
 }
 
-template <typename REAL, int M, bool CONTIG, bool GRAD>
-hipError_t launch_adjoint_m(const Geom &g, const HybridCfg &cfg, const REAL *a,
-                            const REAL *b, const REAL *c, const REAL *x, const REAL *w,
-                            REAL *lam, REAL *ga, REAL *gb, REAL *gc, hipStream_t st) {
+// Nontemporal loads and stores except for strided systems whose rows are
+// 64 KiB or more apart (256^3 FP64, all gradients: x / y 0.233 / 0.233 ms
+// with them against 0.242 / 0.252, z 0.294 against 0.282).
+template <typename REAL, int M, bool CONTIG, bool GRAD, bool NT>
+hipError_t launch_adjoint_nt(const Geom &g, const HybridCfg &cfg, const REAL *a,
+                             const REAL *b, const REAL *c, const REAL *x, const REAL *w,
+                             REAL *lam, REAL *ga, REAL *gb, REAL *gc, hipStream_t st) {
   const int T = cfg.W * cfg.P;
   size_t lds = hyb::ring_bytes<REAL>(T, 3);
   if (CONTIG) lds += sizeof(REAL) * (size_t)T * (M + 1);
-  auto kern = adjoint_kernel<REAL, M, CONTIG, GRAD>;
+  auto kern = adjoint_kernel<REAL, M, CONTIG, GRAD, NT>;
   static LdsGrant lds_granted;
   {
     hipError_t e = ensure_lds(reinterpret_cast<const void *>(kern), lds, lds_granted);
@@ -126,6 +129,17 @@ hipError_t launch_adjoint_m(const Geom &g, const HybridCfg &cfg, const REAL *a,
   hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(T), lds, st, g, cfg.P, cfg.W, cfg.G,
                      a, b, c, x, w, lam, ga, gb, gc);
   return hipGetLastError();
+}
+
+template <typename REAL, int M, bool CONTIG, bool GRAD>
+hipError_t launch_adjoint_m(const Geom &g, const HybridCfg &cfg, const REAL *a,
+                            const REAL *b, const REAL *c, const REAL *x, const REAL *w,
+                            REAL *lam, REAL *ga, REAL *gb, REAL *gc, hipStream_t st) {
+  if (CONTIG || g.stride * (int64_t)sizeof(REAL) < (64 << 10))
+    return launch_adjoint_nt<REAL, M, CONTIG, GRAD, true>(g, cfg, a, b, c, x, w, lam, ga, gb,
+                                                          gc, st);
+  return launch_adjoint_nt<REAL, M, CONTIG, GRAD, false>(g, cfg, a, b, c, x, w, lam, ga, gb,
+                                                         gc, st);
 }
 
 template <typename REAL>

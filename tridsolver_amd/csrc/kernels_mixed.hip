@@ -46,7 +46,7 @@ __global__ __launch_bounds__(MIXED_THREADS) void mixed_kernel(
   double *tile = reinterpret_cast<double *>(smem_raw) + (size_t)L.w * P * (M + 1);
   hyb::Ring ring{smem_raw + tile_bytes, (int)(3 * sizeof(double) * L.T), 0};
   double A[M], B[M], Cc[M], D[M];
-  hyb::fetch_abcd<M, CONTIG>(L, a, b, c, d, A, B, Cc, D);
+  hyb::fetch_abcd<M, CONTIG, true>(L, a, b, c, d, A, B, Cc, D);
   hyb::to_chunk<M, CONTIG>(L, A, tile);
   hyb::to_chunk<M, CONTIG>(L, B, tile);
   hyb::to_chunk<M, CONTIG>(L, Cc, tile);
@@ -55,7 +55,7 @@ __global__ __launch_bounds__(MIXED_THREADS) void mixed_kernel(
   double X[M];
   mixed_chunk<M>(L, ring, A, B, Cc, D, refine, X);
   hyb::to_fetch<M, CONTIG>(L, X, tile);
-  hyb::store<M, CONTIG>(L, d, X);
+  hyb::store<M, CONTIG, true>(L, d, X);
 }
 
 template <int M, bool CONTIG>
@@ -103,7 +103,7 @@ __global__ __launch_bounds__(MIXED_THREADS) void mixed_tile_kernel(
   double *tC = tA + (size_t)L.T * (M + 1);
   hyb::Ring ring{smem_raw + 2 * sizeof(double) * (size_t)L.T * (M + 1),
                  (int)(2 * sizeof(double) * L.T), 0};
-  mixed_contig_lds<M, MIXED_MAX_STEPS>(L, ring, a, b, c, d, tA, tC, refine, [] {},
+  mixed_contig_lds<M, MIXED_MAX_STEPS, true>(L, ring, a, b, c, d, tA, tC, refine, [] {},
                                        [&] { return L.base; });
 }
 
@@ -159,7 +159,7 @@ __global__ __launch_bounds__(MIXED_THREADS) void mixed_wide_kernel(
   const uint32_t ob = (uint32_t)((L.base + row0 * L.stride) * (int64_t)sizeof(double));
   const uint32_t os = (uint32_t)(L.stride * (int64_t)sizeof(double));
   auto at = [](const double *p, uint32_t off) -> double {
-    return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(p) + off);
+    return hyb::ld<true>(reinterpret_cast<const double *>(reinterpret_cast<const char *>(p) + off));
   };
   const bool full = L.active && row0 + M <= L.N;
   double B[M], D[M];
@@ -205,7 +205,7 @@ __global__ __launch_bounds__(MIXED_THREADS) void mixed_wide_kernel(
 #pragma unroll
   for (int r = 0; r < M; ++r) {
     if (r < rows)
-      *reinterpret_cast<double *>(reinterpret_cast<char *>(d) + obs + (uint32_t)r * os) = X[r];
+      hyb::st<true>(reinterpret_cast<double *>(reinterpret_cast<char *>(d) + obs + (uint32_t)r * os), X[r]);
   }
 }
 

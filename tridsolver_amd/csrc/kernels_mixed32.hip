@@ -62,7 +62,7 @@ __global__ __launch_bounds__(M32_THREADS) void mixed32_kernel(
   float *ftile = reinterpret_cast<float *>(dtile);
   hyb::Ring ring{smem_raw + tile_bytes, (int)(2 * sizeof(double) * L.T), 0};
   float A[M], B[M], Cc[M], D[M];
-  hyb::fetch_abcd<M, CONTIG>(L, a, b, c, d, A, B, Cc, D);
+  hyb::fetch_abcd<M, CONTIG, true>(L, a, b, c, d, A, B, Cc, D);
   hyb::to_chunk<M, CONTIG>(L, A, ftile);
   hyb::to_chunk<M, CONTIG>(L, B, ftile);
   hyb::to_chunk<M, CONTIG>(L, Cc, ftile);
@@ -71,7 +71,7 @@ __global__ __launch_bounds__(M32_THREADS) void mixed32_kernel(
   mixed_chunk_acc<M, MIXED_MAX_STEPS, false, ONE, true>(
       L, ring, [&](int r) { return A[r]; }, B, [&](int r) { return Cc[r]; }, D, refine, X);
   hyb::to_fetch<M, CONTIG>(L, X, dtile);
-  hyb::store<M, CONTIG>(L, x, X);
+  hyb::store<M, CONTIG, true>(L, x, X);
 }
 
 // a and c kept in LDS, read from there by the solve, which frees 2*M
@@ -94,7 +94,7 @@ __global__ __launch_bounds__(M32_THREADS) void mixed32_lds_kernel(
     hyb::Ring ring{smem_raw + 2 * sizeof(float) * (size_t)M * T, (int)(2 * sizeof(double) * T),
                    0};
     float A[M], B[M], Cc[M], D[M];
-    hyb::fetch_abcd<M, false>(L, a, b, c, d, A, B, Cc, D);
+    hyb::fetch_abcd<M, false, true>(L, a, b, c, d, A, B, Cc, D);
 #pragma unroll
     for (int r = 0; r < M; ++r) {
       tA[r * T + t] = A[r];
@@ -107,7 +107,7 @@ __global__ __launch_bounds__(M32_THREADS) void mixed32_lds_kernel(
     mixed_chunk_acc<M, STEPS, false, ONE, true>(
         L, ring, [&](int r) { return tA[r * T + t]; }, B, [&](int r) { return tC[r * T + t]; },
         D, refine, X);
-    hyb::store<M, false>(L, x, X);
+    hyb::store<M, false, true>(L, x, X);
     return;
   }
   const int tile = P * (M + 1);
@@ -117,7 +117,7 @@ __global__ __launch_bounds__(M32_THREADS) void mixed32_lds_kernel(
                  (int)(2 * sizeof(double) * L.T), 0};
   {
     float A[M], B[M], Cc[M], D[M];
-    hyb::fetch_abcd<M, true>(L, a, b, c, d, A, B, Cc, D);
+    hyb::fetch_abcd<M, true, true>(L, a, b, c, d, A, B, Cc, D);
     hyb::to_chunk<M, true>(L, B, tA);
     hyb::to_chunk<M, true>(L, D, tA);
 #pragma unroll
@@ -133,7 +133,7 @@ __global__ __launch_bounds__(M32_THREADS) void mixed32_lds_kernel(
         L, ring, [&](int r) { return ca[r]; }, B, [&](int r) { return cc[r]; }, D, refine, X);
     __syncthreads();  // a and c are read to the end before x overwrites the tile
     hyb::to_fetch<M, true>(L, X, dtile);
-    hyb::store<M, true>(L, x, X);
+    hyb::store<M, true, true>(L, x, X);
   }
 }
 
@@ -369,6 +369,8 @@ hipError_t launch_wide32_one(const Geom &g, const HybridCfg &cfg, int refine, co
                              hipStream_t st) {
   const int T = cfg.W * cfg.P;
   const size_t lds = wide_lds_bytes(cfg);
+  // plain loads and stores: with nontemporal ones z (512^3) took 0.90 ms
+  // against 0.81
   auto kern = mixed32_wide_kernel<M, W, ONE>;
   static LdsGrant lds_granted;
   {

@@ -647,7 +647,7 @@ __global__ __launch_bounds__(T, 2) void rg_solve_mixed_lds(
     int nxt = 0;
     RgMeta next{};
     // at most T lanes per system: log2(T) PCR steps
-    mixed_contig_lds<M, T == 512 ? 9 : 8>(L, ring, a, b, c, d, tA, tC, refine, [&] {
+    mixed_contig_lds<M, T == 512 ? 9 : 8, false>(L, ring, a, b, c, d, tA, tC, refine, [&] {
       nxt = s_next;
       if (nxt < total) next = rg_meta(s_view, list, off, nxt, t, T);
     }, [&] { return s_base[t]; });

@@ -167,13 +167,13 @@ __global__ __launch_bounds__((HybridBounds<REAL, M>::threads)) void hybrid_kerne
 
   REAL ra[M], rb[M], rc[M], rd[M];
   if constexpr (VEC) {
-    hyb::vfetch_abcd<M, V>(L, a, b, c, d, ra, rb, rc, rd);
+    hyb::vfetch_abcd<M, V, true>(L, a, b, c, d, ra, rb, rc, rd);
     hyb::vto_chunk<M, V>(L, ra, tile);
     hyb::vto_chunk<M, V>(L, rb, tile);
     hyb::vto_chunk<M, V>(L, rc, tile);
     hyb::vto_chunk<M, V>(L, rd, tile);
   } else {
-    hyb::fetch_abcd<M, CONTIG>(L, a, b, c, d, ra, rb, rc, rd);
+    hyb::fetch_abcd<M, CONTIG, true>(L, a, b, c, d, ra, rb, rc, rd);
     hyb::to_chunk<M, CONTIG>(L, ra, tile);
     hyb::to_chunk<M, CONTIG>(L, rb, tile);
     hyb::to_chunk<M, CONTIG>(L, rc, tile);
@@ -183,11 +183,11 @@ __global__ __launch_bounds__((HybridBounds<REAL, M>::threads)) void hybrid_kerne
   hyb::solve_chunk<REAL, M, CONTIG>(L, ring, ra, rb, rc, rd, x);
   if constexpr (VEC) {
     hyb::vto_fetch<M, V>(L, x, tile);
-    hyb::vstore<M, V, INC>(L, INC ? u : d, x);
+    hyb::vstore<M, V, INC, true>(L, INC ? u : d, x);
   } else {
     hyb::to_fetch<M, CONTIG>(L, x, tile);
     if (INC) hyb::store_add<M, CONTIG>(L, u, x);
-    else hyb::store<M, CONTIG>(L, d, x);
+    else hyb::store<M, CONTIG, true>(L, d, x);
   }
 }
 

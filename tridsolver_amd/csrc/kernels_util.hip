@@ -13,7 +13,9 @@
 //                        d into registers, d written back), with one add per
 //                        value in place of the solve: the bandwidth that access
 //                        pattern allows, the ceiling the solve's rate is
-//                        compared with.
+//                        compared with. access_probe_x_kernel does the same
+//                        for contiguous (x) systems. The probes' b, c, d and
+//                        stores are nontemporal, as the solves' are.
 #include "../../include/tridsolver_hip.h"
 #include <algorithm>
 
@@ -113,7 +115,7 @@ __global__ __launch_bounds__(M == 32 ? 512 : 1024) void access_probe_kernel(
   const uint32_t ob = (uint32_t)((L.base + row0 * L.stride) * 8);
   const uint32_t os = (uint32_t)(L.stride * 8);
   auto at = [](const double *p, uint32_t off) -> double {
-    return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(p) + off);
+    return hyb::ld<true>(reinterpret_cast<const double *>(reinterpret_cast<const char *>(p) + off));
   };
   const int rows = L.active ? (int)std::min<int64_t>(M, std::max<int64_t>(0, L.N - row0)) : 0;
   double rb[M], rc[M], rd[M];
@@ -135,7 +137,7 @@ __global__ __launch_bounds__(M == 32 ? 512 : 1024) void access_probe_kernel(
     if (r < rows) {
       double *dst = reinterpret_cast<double *>(reinterpret_cast<char *>(d) + obs +
                                                (uint32_t)r * os);
-      *dst = rb[r] + rc[r] + rd[r] + al[r * W];
+      hyb::st<true>(dst, rb[r] + rc[r] + rd[r] + al[r * W]);
     }
   }
 }
@@ -169,9 +171,12 @@ __global__ __launch_bounds__(256) void access_probe_x_kernel(Geom g, int P, int 
   for (int r = 0; r < M; ++r) {
     const int64_t row = hyb::fetch_row<M, true>(L, r);
     const int64_t e = L.base + row;
-    v[r] = (L.active && row < L.N) ? a[e] + b[e] + c[e] + d[e] : 0.0;
+    v[r] = (L.active && row < L.N)
+               ? hyb::ld<true>(a + e) + hyb::ld<true>(b + e) + hyb::ld<true>(c + e) +
+                     hyb::ld<true>(d + e)
+               : 0.0;
   }
-  hyb::store<M, true>(L, d, v);
+  hyb::store<M, true, true>(L, d, v);
 }
 
 template <int M>

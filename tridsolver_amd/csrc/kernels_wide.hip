@@ -102,7 +102,7 @@ __global__ __launch_bounds__((WideBounds<REAL, M>::threads)) void hybrid_wide_ke
   const uint32_t ob = (uint32_t)((L.base + row0 * L.stride) * (int64_t)sizeof(REAL));
   const uint32_t os = (uint32_t)(L.stride * (int64_t)sizeof(REAL));
   auto at = [](const REAL *p, uint32_t off) -> REAL {
-    return *reinterpret_cast<const REAL *>(reinterpret_cast<const char *>(p) + off);
+    return hyb::ld<true>(reinterpret_cast<const REAL *>(reinterpret_cast<const char *>(p) + off));
   };
   // Lanes whose M rows all lie inside the system (every lane when N = P*M)
   // load without per-row masks.
@@ -158,7 +158,7 @@ __global__ __launch_bounds__((WideBounds<REAL, M>::threads)) void hybrid_wide_ke
       REAL *dst = reinterpret_cast<REAL *>(reinterpret_cast<char *>(INC ? u : d) + obs +
                                            (uint32_t)r * os);
       if (INC) *dst += x;
-      else *dst = x;
+      else hyb::st<true>(dst, x);
     }
     __builtin_amdgcn_sched_barrier(0);
   }

@@ -243,10 +243,11 @@ __device__ __forceinline__ void mixed_chunk(const hyb::Lane &L, hyb::Ring &ring,
 // Rows are 32-bit here (p + P*r < P*M <= 8192) and each array is addressed
 // from one per-lane pointer, so no 64-bit row index per row stays live (the
 // shared fetch's would, through to the store, and spill at this register
-// pressure). `store_base()` gives L.base again for the stores: a caller
-// whose register file is full returns it from an LDS slot, so that no
-// 64-bit base stays live across the solve.
-template <int M, int STEPS, typename OnFetched, typename StoreBase>
+// pressure). NT: nontemporal loads and stores (hyb::ld / hyb::st).
+// `store_base()` gives L.base again for the stores: a caller whose register
+// file is full returns it from an LDS slot, so that no 64-bit base stays
+// live across the solve.
+template <int M, int STEPS, bool NT, typename OnFetched, typename StoreBase>
 __device__ __forceinline__ void mixed_contig_lds(const hyb::Lane &L, hyb::Ring &ring,
                                                  const double *__restrict__ a,
                                                  const double *__restrict__ b,
@@ -263,10 +264,10 @@ __device__ __forceinline__ void mixed_contig_lds(const hyb::Lane &L, hyb::Ring &
     for (int r = 0; r < M; ++r) {
       const int row = p + P * r;
       const bool ok = row < N;
-      A[r] = (ok && row > 0) ? pa[P * r] : 0.0;
-      B[r] = ok ? pb[P * r] : 1.0;
-      Cc[r] = (row < N - 1) ? pc[P * r] : 0.0;
-      D[r] = ok ? pd[P * r] : 0.0;
+      A[r] = (ok && row > 0) ? hyb::ld<NT>(pa + P * r) : 0.0;
+      B[r] = ok ? hyb::ld<NT>(pb + P * r) : 1.0;
+      Cc[r] = (row < N - 1) ? hyb::ld<NT>(pc + P * r) : 0.0;
+      D[r] = ok ? hyb::ld<NT>(pd + P * r) : 0.0;
     }
   }
   hyb::to_chunk<M, true>(L, B, tA);
@@ -291,7 +292,7 @@ __device__ __forceinline__ void mixed_contig_lds(const hyb::Lane &L, hyb::Ring &
   double *qd = d + sb;
 #pragma unroll
   for (int r = 0; r < M; ++r)
-    if (p + P * r < N) qd[P * r] = X[r];
+    if (p + P * r < N) hyb::st<NT>(qd + P * r, X[r]);
 }
 
 }  // namespace trid
